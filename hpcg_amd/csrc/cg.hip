@@ -1,0 +1,380 @@
+// Preconditioned CG on the device.  One iteration of the fp64 variant is the
+// MG V-cycle plus four fused kernels:
+//   rz      rtz = r.z ; beta = rtz/oldrtz                 (CG.cpp:110-114)
+//   p       p = z + beta p   (k == 1: p = z + 0 z)         (CG.cpp:109,115)
+//   spmv    Ap = A p ; pAp = p.Ap ; alpha = rtz/pAp        (CG.cpp:118-120)
+//   update  x += alpha p ; r -= alpha Ap ; normr2 = r.r    (CG.cpp:121-123)
+// The scalars never leave the device; the host reads normr2 once per
+// iteration for the stopping test.  Element-wise arithmetic is the
+// reference's (no FMA); only the dot products' summation order differs.
+#include <chrono>
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace hpcgx {
+
+static inline unsigned tiles(int64_t rows) { return (unsigned)((rows + kBlock - 1) / kBlock); }
+
+template <class ZT>
+__global__ __launch_bounds__(kBlock) void k_cg_rz(int64_t n, const double *__restrict__ r,
+                                                  const ZT *__restrict__ z, double *partials,
+                                                  unsigned *ticket, double *scal, int first) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    acc += r[i] * (double)z[i];
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, ticket, tot)) {
+    if (first) {
+      scal[S_RTZ] = tot[0];
+    } else {
+      const double old = scal[S_RTZ];
+      scal[S_OLDRTZ] = old;
+      scal[S_RTZ] = tot[0];
+      scal[S_BETA] = tot[0] / old;
+    }
+  }
+}
+
+// p = z + beta p  (first: z + 0.0 z, the WAXPBY(1,z,0,z) of CG.cpp:109)
+template <class ZT>
+__global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict__ z, double *__restrict__ p,
+                                                 const double *__restrict__ scal, int first) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double zi = (double)z[i];
+  p[i] = first ? zi + 0.0 * zi : zi + scal[S_BETA] * p[i];
+}
+
+// Ap = A p and pAp = p.Ap ; alpha = rtz / pAp.  Grid-stride over 256-row
+// tiles with a fixed grid, so the reduction order is fixed.
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const int *__restrict__ cols,
+                                                        const double *__restrict__ vals,
+                                                        const double *__restrict__ p,
+                                                        double *__restrict__ Ap, int64_t nrows,
+                                                        double *partials, unsigned *ticket, double *scal) {
+  double acc = 0.0;
+  const int64_t ntiles = (nrows + kBlock - 1) / kBlock;
+  for (int64_t tile = tile_id(); tile < ntiles; tile += gridDim.x) {
+    const int64_t row = tile * kBlock + threadIdx.x;
+    if (row < nrows) {
+      const size_t b = ell_base(row);
+      double sum = 0.0;
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s) {
+        const int c = ld_col(cols + b + s * kSlice);
+        const double v = ld_val<double>(vals + b + s * kSlice);
+        sum += v * p[c];
+      }
+      Ap[row] = sum;
+      acc += p[row] * sum;
+    }
+  }
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, ticket, tot)) {
+    scal[S_PAP] = tot[0];
+    scal[S_ALPHA] = scal[S_RTZ] / tot[0];
+  }
+}
+
+// x += alpha p ; r -= alpha Ap ; normr2 = r.r ; hist[k] = normr2
+// R32: also leave r in fp32 for the next mixed-precision V-cycle.
+template <bool R32>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restrict__ x,
+                                                      const double *__restrict__ p, double *__restrict__ r,
+                                                      const double *__restrict__ Ap, float *__restrict__ r32,
+                                                      double *partials, unsigned *ticket, double *scal,
+                                                      double *hist, int k) {
+  const double alpha = scal[S_ALPHA], malpha = -alpha;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] + malpha * Ap[i];
+    r[i] = ri;
+    if (R32) r32[i] = (float)ri;
+    acc += ri * ri;
+  }
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, ticket, tot)) {
+    scal[S_NORMR2] = tot[0];
+    if (hist) hist[k] = tot[0];
+  }
+}
+
+// r = b - Ap ; normr2 = r.r (CG.cpp:88-89) ; optional fp32 copy of r
+template <bool R32>
+__global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double *__restrict__ b,
+                                                        const double *__restrict__ Ap, double *__restrict__ r,
+                                                        float *__restrict__ r32, double *partials,
+                                                        unsigned *ticket, double *scal) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double ri = b[i] + -1.0 * Ap[i];
+    r[i] = ri;
+    if (R32) r32[i] = (float)ri;
+    acc += ri * ri;
+  }
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, ticket, tot)) scal[S_NORMR2] = tot[0];
+}
+
+// Flexible CG (Polak-Ribiere): rtz = r.z ; zAp = z.Ap_{k-1} ;
+// beta = -alpha_{k-1} zAp / oldrtz, since r_k - r_{k-1} = -alpha_{k-1} A p_{k-1}.
+__global__ __launch_bounds__(kBlock) void k_fcg_rz(int64_t n, const double *__restrict__ r,
+                                                   const float *__restrict__ z, const double *__restrict__ Ap,
+                                                   double *partials, unsigned *ticket, double *scal,
+                                                   int restart) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double a0 = 0.0, a1 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double zi = (double)z[i];
+    a0 += r[i] * zi;
+    if (!restart) a1 += zi * Ap[i];
+  }
+  double v[2] = {a0, a1}, tot[2];
+  if (grid_reduce<2>(v, partials, ticket, tot)) {
+    if (restart) {
+      scal[S_RTZ] = tot[0];
+    } else {
+      const double old = scal[S_RTZ];
+      scal[S_OLDRTZ] = old;
+      scal[S_RTZ] = tot[0];
+      scal[S_BETA] = -scal[S_ALPHA] * tot[1] / old;
+    }
+  }
+}
+
+// x[...] copy for p = z on restart (fp32 z)
+__global__ __launch_bounds__(kBlock) void k_copy_z(int64_t n, const float *__restrict__ z, double *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = (double)z[i];
+}
+
+// dL/dkappa (see hpcgx_kappa_grad); neighbours in ascending slot order
+__global__ __launch_bounds__(kBlock) void k_kappa_grad(const int *__restrict__ cols, const int *__restrict__ iperm,
+                                                       const double *__restrict__ lam,
+                                                       const double *__restrict__ x, double *__restrict__ g,
+                                                       int64_t npad) {
+  const int64_t p = tile_id() * kBlock + threadIdx.x;
+  if (p >= npad) return;
+  if (iperm[p] < 0) {
+    g[p] = 0.0;
+    return;
+  }
+  const size_t b = ell_base(p);
+  const double lp = lam[p], xp = x[p];
+  double acc = 0.0;
+  int nn = 0;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    if (s == kDiagSlot) continue;
+    const int c = ld_col(cols + b + s * kSlice);
+    if (c == (int)p) continue;
+    acc += (lp - lam[c]) * (xp - x[c]);
+    ++nn;
+  }
+  g[p] = -0.5 * acc - (double)(26 - nn) * lp * xp;
+}
+
+static double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static int ensure_hist(Ctx *c, int max_iter) {
+  if (c->hist_cap >= max_iter + 1) return HPCGX_OK;
+  if (c->hist) dfree(c, c->hist, c->hist_cap * sizeof(double));
+  c->hist = nullptr;
+  c->hist_cap = 0;
+  int rc = dalloc(c, (void **)&c->hist, (size_t)(max_iter + 1) * sizeof(double));
+  if (rc) return rc;
+  c->hist_cap = max_iter + 1;
+  return HPCGX_OK;
+}
+
+static int read_scalar(Ctx *c, int which, double *out) {
+  HPCGX_HIP(c, hipMemcpyAsync(c->hbuf, c->scal + which, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  *out = c->hbuf[0];
+  return HPCGX_OK;
+}
+
+// Per-phase timing with events recorded between the phases of an iteration
+// (times[1] dot, [2] WAXPBY, [3] SpMV, [5] MG; the fused update kernel is
+// counted as WAXPBY).
+struct PhaseTimer {
+  Ctx *c;
+  bool on;
+  double acc[6] = {};
+  void mark(int i) {
+    if (on) (void)hipEventRecord(c->ev[i], c->stream);
+  }
+  void collect() {  // after a stream sync: ev0..ev5 = MG | rz | p | spmv | update
+    if (!on) return;
+    float ms;
+    static const int slot[5] = {5, 1, 2, 3, 2};
+    for (int i = 0; i < 5; ++i)
+      if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) acc[slot[i]] += ms * 1e-3;
+  }
+};
+
+static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol, bool precond, int *niters,
+                   double *normr_out, double *normr0_out, double *times, double *hist) {
+  const double t_begin = now_s();
+  Level &L0 = c->levels[0];
+  const int64_t np = L0.lay.npad;
+  const int G = reduce_grid_size(np);
+  double *r = c->cg_r, *z = precond ? c->cg_z : c->cg_r, *p = c->cg_p, *Ap = c->cg_Ap;
+  int rc;
+  PhaseTimer tm{c, times != nullptr};
+  // preamble: Ap = A x ; r = b - Ap ; normr = ||r||  (CG.cpp:86-90)
+  if ((rc = spmv_fp64(c, 0, x, Ap))) return rc;
+  k_cg_residual<false><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, nullptr, c->partials, c->ticket, c->scal);
+  HPCGX_CHECK_LAUNCH(c);
+  double n2;
+  if ((rc = read_scalar(c, S_NORMR2, &n2))) return rc;
+  double normr = std::sqrt(n2);
+  const double normr0 = normr;
+  if (hist) hist[0] = normr;
+  *niters = 0;
+  for (int k = 1; k <= max_iter && normr / normr0 > tol; ++k) {
+    tm.mark(0);
+    if (precond && (rc = mg_fp64(c, 0, r, z))) return rc;
+    tm.mark(1);
+    k_cg_rz<double><<<G, kBlock, 0, c->stream>>>(np, r, z, c->partials, c->ticket, c->scal, k == 1);
+    tm.mark(2);
+    k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, k == 1);
+    tm.mark(3);
+    k_cg_spmv_dot<<<G, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials, c->ticket, c->scal);
+    tm.mark(4);
+    k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->ticket, c->scal,
+                                                    nullptr, k);
+    tm.mark(5);
+    HPCGX_CHECK_LAUNCH(c);
+    if ((rc = read_scalar(c, S_NORMR2, &n2))) return rc;
+    tm.collect();
+    normr = std::sqrt(n2);
+    if (hist) hist[k] = normr;
+    *niters = k;
+  }
+  *normr_out = normr;
+  *normr0_out = normr0;
+  if (times) {
+    for (int i = 1; i < 6; ++i) times[i] += tm.acc[i];
+    times[0] += now_s() - t_begin;
+  }
+  return HPCGX_OK;
+}
+
+static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_iter, double tol, int *niters,
+                    double *normr_out, double *normr0_out, double *times, double *hist) {
+  const double t_begin = now_s();
+  Level &L0 = c->levels[0];
+  const int64_t np = L0.lay.npad;
+  const int G = reduce_grid_size(np);
+  double *r = c->cg_r, *p = c->cg_p, *Ap = c->cg_Ap;
+  float *r32 = c->cg_r32, *z32 = c->cg_z32;
+  int rc;
+  PhaseTimer tm{c, times != nullptr};
+  auto true_residual = [&](double *n2) -> int {
+    int e = spmv_fp64(c, 0, x, Ap);
+    if (e) return e;
+    k_cg_residual<true><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, r32, c->partials, c->ticket, c->scal);
+    HPCGX_CHECK_LAUNCH(c);
+    return read_scalar(c, S_NORMR2, n2);
+  };
+  double n2;
+  if ((rc = true_residual(&n2))) return rc;
+  double normr = std::sqrt(n2);
+  const double normr0 = normr;
+  if (hist) hist[0] = normr;
+  *niters = 0;
+  bool restart = true;
+  int k = 0;
+  while (k < max_iter) {
+    if (!(normr / normr0 > tol)) {
+      if ((rc = true_residual(&n2))) return rc;
+      normr = std::sqrt(n2);
+      if (!(normr / normr0 > tol)) break;
+      restart = true;
+    }
+    ++k;
+    tm.mark(0);
+    if (variant == HPCGX_CG_MIXED_FP16)
+      rc = mg_fp16(c, r32, z32);
+    else
+      rc = mg_fp32(c, 0, r32, z32);
+    if (rc) return rc;
+    tm.mark(1);
+    k_fcg_rz<<<G, kBlock, 0, c->stream>>>(np, r, z32, Ap, c->partials, c->ticket, c->scal, restart ? 1 : 0);
+    tm.mark(2);
+    if (restart)
+      k_copy_z<<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p);
+    else
+      k_cg_p<float><<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, c->scal, 0);
+    restart = false;
+    tm.mark(3);
+    k_cg_spmv_dot<<<G, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials, c->ticket, c->scal);
+    tm.mark(4);
+    k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->ticket, c->scal,
+                                                   nullptr, k);
+    tm.mark(5);
+    HPCGX_CHECK_LAUNCH(c);
+    if ((rc = read_scalar(c, S_NORMR2, &n2))) return rc;
+    tm.collect();
+    normr = std::sqrt(n2);
+    if (hist) hist[k] = normr;
+    *niters = k;
+  }
+  *normr_out = normr;
+  *normr0_out = normr0;
+  if (times) {
+    for (int i = 1; i < 6; ++i) times[i] += tm.acc[i];
+    times[0] += now_s() - t_begin;
+  }
+  return HPCGX_OK;
+}
+
+}  // namespace hpcgx
+
+using namespace hpcgx;
+
+extern "C" {
+
+int hpcgx_cg(hpcgx_ctx *c, int variant, const double *b, double *x, int max_iter, double tol,
+             int do_preconditioning, int *niters, double *normr, double *normr0, double *times,
+             double *hist) {
+  if (!c) return HPCGX_EINVAL;
+  if (!b || !x || !niters || !normr || !normr0 || max_iter < 0 || b == x)
+    return set_err(c, HPCGX_EINVAL, "cg: bad argument");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  if (variant == HPCGX_CG_FP64)
+    return cg_fp64(c, b, x, max_iter, tol, do_preconditioning != 0, niters, normr, normr0, times, hist);
+  if (variant == HPCGX_CG_MIXED_FP32 || variant == HPCGX_CG_MIXED_FP16) {
+    const int need = variant == HPCGX_CG_MIXED_FP32 ? 1 : 2;
+    if (!(c->prepared & need))
+      return set_err(c, HPCGX_ESTATE, "cg: call hpcgx_prepare_precision for this variant first");
+    if (!do_preconditioning) return set_err(c, HPCGX_EINVAL, "cg: mixed variants are preconditioned");
+    return cg_mixed(c, variant, b, x, max_iter, tol, niters, normr, normr0, times, hist);
+  }
+  return set_err(c, HPCGX_EINVAL, "cg: unknown variant");
+}
+
+int hpcgx_kappa_grad(hpcgx_ctx *c, const double *lam, const double *x, double *g) {
+  if (!c) return HPCGX_EINVAL;
+  if (!lam || !x || !g) return set_err(c, HPCGX_EINVAL, "kappa_grad: null vector");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  const Level &L = c->levels[0];
+  k_kappa_grad<<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.iperm, lam, x, g, L.lay.npad);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+}  // extern "C"

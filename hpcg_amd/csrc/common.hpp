@@ -1,0 +1,144 @@
+// Internal state of an hpcgx context: per-level device matrices in 27-slot
+// sliced ELL, colour layout, work vectors and reduction scratch.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hpcgx.h"
+
+namespace hpcgx {
+
+constexpr int kSlots = 27;     // one slot per stencil offset (dx,dy,dz) in {-1,0,1}^3
+constexpr int kSlice = 64;     // rows per ELL slice = one wavefront
+constexpr int kDiagSlot = 13;  // slot of offset (0,0,0)
+constexpr int kBlock = 256;    // threads per block = 4 slices
+constexpr int kColours = 8;
+constexpr int kReduceBlocks = 2048;  // fixed grid of reducing kernels (determinism)
+constexpr int kMaxLevels = 8;
+
+// Device scalars of the CG recurrences (Ctx::scal).
+enum Scalar : int {
+  S_RTZ = 0,
+  S_OLDRTZ,
+  S_PAP,
+  S_NORMR2,
+  S_ALPHA,
+  S_BETA,
+  S_DOT,
+  S_DOT2,
+  S_COUNT = 16
+};
+
+// Row layout of one level.  Rows are grouped in 8 blocks, block k holding the
+// rows of parity class 7-k ((x&1)|(y&1)<<1|(z&1)<<2), so the forward sweep
+// visits classes 7,6,...,0 and the injected rows (all coordinates even,
+// class 0) are the last block.  Inside a block rows keep natural order of
+// (x>>1, y>>1, z>>1).  Each block is padded to a multiple of 64 rows; padded
+// rows hold the identity row (diagonal 1, other slots 0) and zero vectors.
+struct Layout {
+  int nx = 0, ny = 0, nz = 0;
+  int64_t n = 0, npad = 0;
+  int64_t start[kColours + 1] = {};  // padded block starts
+  int64_t count[kColours] = {};      // true rows per block
+  int nb[kColours][3] = {};          // block extents in x, y, z
+};
+
+Layout make_layout(int nx, int ny, int nz);
+
+__host__ __device__ inline int parity_class(int ix, int iy, int iz) {
+  return (ix & 1) | ((iy & 1) << 1) | ((iz & 1) << 2);
+}
+
+struct LayoutDev {  // by-value kernel argument
+  int nx, ny, nz;
+  int64_t start[kColours + 1];
+  int nb[kColours][3];
+};
+
+__host__ __device__ inline int64_t perm_of(const LayoutDev &L, int ix, int iy, int iz) {
+  const int c = parity_class(ix, iy, iz);
+  const int k = 7 - c;
+  return L.start[k] + ((int64_t)(iz >> 1) * L.nb[k][1] + (iy >> 1)) * L.nb[k][0] + (ix >> 1);
+}
+
+struct Level {
+  Layout lay;
+  LayoutDev ldev;
+  int64_t nnz = 0;
+  bool kappa = false;
+  // host copies of the permutation
+  std::vector<int> perm_h;   // natural -> colour order [n]
+  std::vector<int> iperm_h;  // colour order -> natural, -1 on padding [npad]
+  std::vector<int> f2c_h;    // natural f2c of this level (uploaded levels), [n_coarse]
+  // device matrix, slot-major inside 64-row slices: (slice*27+slot)*64+lane
+  int *cols = nullptr;
+  double *vals = nullptr;
+  float *vals32 = nullptr;
+  __half *vals16 = nullptr;
+  float *dinv = nullptr;   // fp16 copy: 1/sqrt(a_ii), colour order
+  float *sigma = nullptr;  // fp16 copy: sqrt(d_f/d_c) per injected row
+  int *perm = nullptr;
+  int *iperm = nullptr;
+  int *inj = nullptr;      // injected row t of this level -> row of level+1
+  // work vectors used when this level is the coarse level of a V-cycle
+  double *r = nullptr, *x = nullptr;
+  float *r32 = nullptr, *x32 = nullptr;
+  float *r16 = nullptr;  // scaled rhs of the fp16 V-cycle (level 0 only)
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  std::vector<Level> levels;
+  std::string err;
+  double *partials = nullptr;  // [4 * kReduceBlocks]
+  unsigned *ticket = nullptr;
+  double *scal = nullptr;      // device scalars [S_COUNT]
+  double *hist = nullptr;      // device residual history
+  int hist_cap = 0;
+  double *hbuf = nullptr;      // pinned host scratch
+  // solver buffers (level-0 length)
+  double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
+  float *cg_r32 = nullptr, *cg_z32 = nullptr;
+  double *stage = nullptr;     // natural-order staging for upload/download
+  int64_t stage_len = 0;
+  std::vector<void *> user_vecs;
+  int64_t bytes = 0;
+  bool ready = false;
+  int prepared = 0;  // bit 1: fp32 copies, bit 2: fp16 copies
+  hipEvent_t ev[8] = {};
+};
+
+// error handling ------------------------------------------------------------
+int set_err(Ctx *c, int code, const std::string &msg);
+#define HPCGX_HIP(ctx, call)                                                          \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::hpcgx::set_err((ctx), HPCGX_EHIP,                                      \
+                              std::string(#call " : ") + hipGetErrorString(e_));      \
+  } while (0)
+#define HPCGX_CHECK_LAUNCH(ctx) HPCGX_HIP(ctx, hipGetLastError())
+
+int dalloc(Ctx *c, void **p, size_t bytes, bool zero = true);
+void dfree(Ctx *c, void *p, size_t bytes);
+int ensure_ready(Ctx *c);
+int ensure_stage(Ctx *c, int64_t n);
+
+// launch helpers (implemented in ops.hip)
+int mg_fp64(Ctx *c, int level, const double *r, double *x);
+int mg_fp32(Ctx *c, int level, const float *r, float *x);
+int mg_fp16(Ctx *c, const float *r, float *x);
+int spmv_fp64(Ctx *c, int level, const double *x, double *y);
+int symgs_fp64(Ctx *c, int level, const double *r, double *x);
+int reduce_grid_size(int64_t n);
+
+}  // namespace hpcgx
+
+struct hpcgx_ctx : hpcgx::Ctx {};

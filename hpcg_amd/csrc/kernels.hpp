@@ -1,0 +1,217 @@
+// Device kernels of the PCG iteration phase (gfx950, wave64).
+//
+// Matrix storage: 27-slot sliced ELL.  Slot s of a row is the stencil offset
+// (dx,dy,dz) = (s%3-1, s/3%3-1, s/9-1); a neighbour outside the grid is a
+// zero value whose column is the row itself.  Within a slice of 64 rows the
+// layout is slot-major, so one wavefront reads each slot of its 64 rows as one
+// contiguous 256-byte (index) and 512-byte (fp64 value) run.  Slot order is
+// ascending column order, the order in which the reference sums a row
+// (ComputeSPMV_ref.cpp:67-68); the zero slots add exact zeros, so sums match
+// the reference bit for bit.  The whole library is compiled with
+// -ffp-contract=off: products and sums are rounded separately, as in the
+// reference's x86-64 build.
+#pragma once
+
+#include "common.hpp"
+
+namespace hpcgx {
+
+__device__ __forceinline__ size_t ell_base(int64_t row) {
+  return (size_t)(row >> 6) * (kSlots * kSlice) + (size_t)(row & 63);
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD works on
+// one contiguous range of tiles and neighbouring rows share that XCD's L2.
+__device__ __forceinline__ int64_t tile_id() {
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t per = G >> 3, rem = G & 7, xcd = b & 7, idx = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// matrix loads: streamed once per pass, kept out of the way of the vectors
+__device__ __forceinline__ int ld_col(const int *p) { return __builtin_nontemporal_load(p); }
+template <class ACC>
+__device__ __forceinline__ ACC ld_val(const double *p) {
+  return (ACC)__builtin_nontemporal_load(p);
+}
+template <class ACC>
+__device__ __forceinline__ ACC ld_val(const float *p) {
+  return (ACC)__builtin_nontemporal_load(p);
+}
+template <class ACC>
+__device__ __forceinline__ ACC ld_val(const __half *p) {
+  const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+  return (ACC)__half2float(__ushort_as_half(bits));
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reductions: fixed partition, fixed tree, last block finishes
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Reduces NV values over the whole grid.  Returns true in thread 0 of the
+// block that finished last, with the grid totals in tot[].  Partials are
+// summed in block order by every launch, so results do not depend on timing.
+template <int NV>
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, unsigned *ticket,
+                                            double (&tot)[NV]) {
+  __shared__ double sh[4][NV];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) sh[w][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double s = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
+      __hip_atomic_store(&partials[(size_t)k * gridDim.x + blockIdx.x], s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock)
+      s += __hip_atomic_load(&partials[(size_t)k * gridDim.x + i], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    s = wave_sum(s);
+    if (lane == 0) sh[w][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return false;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
+  __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// sparse kernels (one thread per row, 4 slices per block)
+
+// y = A x over rows [row0, row0 + nrows); ComputeSPMV_ref.cpp:61-70
+template <class VT, class XT, class ACC>
+__global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ cols,
+                                                 const VT *__restrict__ vals,
+                                                 const XT *__restrict__ x, XT *__restrict__ y,
+                                                 int64_t row0, int64_t nrows) {
+  const int64_t t = tile_id() * kBlock + threadIdx.x;
+  if (t >= nrows) return;
+  const int64_t row = row0 + t;
+  const size_t b = ell_base(row);
+  ACC sum = 0;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const int c = ld_col(cols + b + s * kSlice);
+    const ACC v = ld_val<ACC>(vals + b + s * kSlice);
+    sum += v * (ACC)x[c];
+  }
+  y[row] = (XT)sum;
+}
+
+// Slots whose neighbour lies in a block swept before block k (block k holds
+// parity class 7-k; the neighbour at offset d has class c ^ oddness(d)).
+__host__ __device__ constexpr uint32_t swept_before_mask(int k) {
+  uint32_t m = 0;
+  for (int s = 0; s < kSlots; ++s) {
+    const int odd = ((s % 3) != 1) | ((((s / 3) % 3) != 1) << 1) | (((s / 9) != 1) << 2);
+    if (7 - ((7 - k) ^ odd) < k) m |= 1u << s;
+  }
+  return m;
+}
+
+// One Gauss-Seidel step on the rows of one colour block; ComputeSYMGS_ref.cpp:72-79:
+// sum = r_i - sum_j a_ij x_j (diagonal included) + x_i a_ii ; x_i = sum / a_ii.
+// ZK >= 0: the step of block ZK in the first forward sweep from x = 0 -- only
+// neighbours in blocks swept before ZK can be nonzero; the other terms are
+// exact zeros and are not read, and neither is x_i.  The skipped set is a
+// compile-time constant, so the remaining loads issue back to back.
+template <class VT, class XT, class ACC, int ZK>
+__global__ __launch_bounds__(kBlock) void k_gs(const int *__restrict__ cols,
+                                               const VT *__restrict__ vals,
+                                               const XT *__restrict__ r, XT *__restrict__ x,
+                                               int64_t row0, int64_t nrows) {
+  constexpr uint32_t mask = ZK >= 0 ? swept_before_mask(ZK) : 0x7ffffffu;
+  const int64_t t = tile_id() * kBlock + threadIdx.x;
+  if (t >= nrows) return;
+  const int64_t row = row0 + t;
+  const size_t b = ell_base(row);
+  const ACC d = ld_val<ACC>(vals + b + kDiagSlot * kSlice);
+  ACC sum = (ACC)r[row];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    if (!((mask >> s) & 1u)) continue;
+    const int c = ld_col(cols + b + s * kSlice);
+    const ACC v = ld_val<ACC>(vals + b + s * kSlice);
+    sum -= v * (ACC)x[c];
+  }
+  if (ZK < 0) sum += (ACC)x[row] * d;
+  x[row] = (XT)(sum / d);
+}
+
+// Residual at the injected rows and restriction, fused (ComputeMG_ref.cpp:49-51 +
+// ComputeRestriction_ref.cpp:51): rc[inj[t]] = r_f - (A x)_f for the t-th
+// injected row f = row0 + t.  (A x)_f is summed from 0 like SpMV, then
+// subtracted.  SCALE: fp16 V-cycle, times sigma[t].
+template <class VT, class XT, class ACC, bool SCALE>
+__global__ __launch_bounds__(kBlock) void k_restrict(const int *__restrict__ cols,
+                                                     const VT *__restrict__ vals,
+                                                     const XT *__restrict__ r,
+                                                     const XT *__restrict__ x,
+                                                     XT *__restrict__ rc, const int *__restrict__ inj,
+                                                     const float *__restrict__ sigma, int64_t row0,
+                                                     int64_t count) {
+  const int64_t t = tile_id() * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int64_t row = row0 + t;
+  const size_t b = ell_base(row);
+  ACC ax = 0;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const int c = ld_col(cols + b + s * kSlice);
+    const ACC v = ld_val<ACC>(vals + b + s * kSlice);
+    ax += v * (ACC)x[c];
+  }
+  ACC res = (ACC)r[row] - ax;
+  if (SCALE) res = res * (ACC)sigma[t];
+  rc[inj[t]] = (XT)res;
+}
+
+// x_f += xc at the injected rows (ComputeProlongation_ref.cpp:49).
+template <class XT, class ACC, bool SCALE>
+__global__ __launch_bounds__(kBlock) void k_prolong(XT *__restrict__ x, const XT *__restrict__ xc,
+                                                    const int *__restrict__ inj,
+                                                    const float *__restrict__ sigma, int64_t row0,
+                                                    int64_t count) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int64_t row = row0 + t;
+  ACC v = (ACC)xc[inj[t]];
+  if (SCALE) v = v * (ACC)sigma[t];
+  x[row] = (XT)((ACC)x[row] + v);
+}
+
+// ---------------------------------------------------------------------------
+// element-wise helpers
+
+// w = alpha x + beta y with the branches of ComputeWAXPBY_ref.cpp:52-67
+__device__ __forceinline__ double waxpby1(double alpha, double x, double beta, double y) {
+  if (alpha == 1.0) return x + beta * y;
+  if (beta == 1.0) return alpha * x + y;
+  return alpha * x + beta * y;
+}
+
+}  // namespace hpcgx

@@ -1,0 +1,296 @@
+// Kernel drivers: SpMV, coloured symmetric Gauss-Seidel, fused residual +
+// restriction, prolongation, the V-cycle in fp64 / fp32 / scaled fp16, and
+// the vector reductions.
+#include "kernels.hpp"
+
+namespace hpcgx {
+
+static inline unsigned tiles(int64_t rows) { return (unsigned)((rows + kBlock - 1) / kBlock); }
+
+template <class VT>
+static const VT *vals_of(const Level &L);
+template <>
+const double *vals_of<double>(const Level &L) { return L.vals; }
+template <>
+const float *vals_of<float>(const Level &L) { return L.vals32; }
+template <>
+const __half *vals_of<__half>(const Level &L) { return L.vals16; }
+
+// One symmetric Gauss-Seidel step over the 8 colour blocks.  Forward sweep
+// blocks 0..7, backward sweep 6..0: backward block 7 would recompute block 7
+// from exactly the neighbour values the forward step used, so it is elided.
+// zero_start: x is known to be zero on entry (the V-cycle's first sweep), so
+// x is not read before it is written.
+template <class VT, class XT, class ACC, int ZK>
+static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, int k) {
+  const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
+  if (nr > 0) k_gs<VT, XT, ACC, ZK><<<tiles(nr), kBlock, 0, c->stream>>>(L.cols, v, r, x, r0, nr);
+}
+
+template <class VT, class XT, class ACC>
+static int symgs_launch(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start) {
+  const VT *v = vals_of<VT>(L);
+  if (zero_start) {
+    gs_block<VT, XT, ACC, 0>(c, L, v, r, x, 0);
+    gs_block<VT, XT, ACC, 1>(c, L, v, r, x, 1);
+    gs_block<VT, XT, ACC, 2>(c, L, v, r, x, 2);
+    gs_block<VT, XT, ACC, 3>(c, L, v, r, x, 3);
+    gs_block<VT, XT, ACC, 4>(c, L, v, r, x, 4);
+    gs_block<VT, XT, ACC, 5>(c, L, v, r, x, 5);
+    gs_block<VT, XT, ACC, 6>(c, L, v, r, x, 6);
+    gs_block<VT, XT, ACC, 7>(c, L, v, r, x, 7);
+  } else {
+    for (int k = 0; k < kColours; ++k) gs_block<VT, XT, ACC, -1>(c, L, v, r, x, k);
+  }
+  for (int k = kColours - 2; k >= 0; --k) gs_block<VT, XT, ACC, -1>(c, L, v, r, x, k);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+// V-cycle of ComputeMG_ref.cpp:39-63: x = 0; pre-smooth; residual + restrict;
+// recurse; prolong; post-smooth.  The coarsest level gets one smoothing step.
+template <class VT, class XT, class ACC, bool SCALE>
+static int vcycle(Ctx *c, int l, const XT *r, XT *x) {
+  const Level &L = c->levels[l];
+  int rc = symgs_launch<VT, XT, ACC>(c, L, r, x, true);
+  if (rc || l + 1 >= (int)c->levels.size()) return rc;
+  Level &C = c->levels[l + 1];
+  XT *rc_v, *xc_v;
+  if constexpr (sizeof(XT) == 8) {
+    rc_v = reinterpret_cast<XT *>(C.r);
+    xc_v = reinterpret_cast<XT *>(C.x);
+  } else {
+    rc_v = reinterpret_cast<XT *>(C.r32);
+    xc_v = reinterpret_cast<XT *>(C.x32);
+  }
+  const int64_t nc = L.lay.count[7], r0 = L.lay.start[7];
+  k_restrict<VT, XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(L.cols, vals_of<VT>(L), r, x, rc_v,
+                                                                       L.inj, L.sigma, r0, nc);
+  HPCGX_CHECK_LAUNCH(c);
+  if ((rc = vcycle<VT, XT, ACC, SCALE>(c, l + 1, rc_v, xc_v))) return rc;
+  k_prolong<XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc_v, L.inj, L.sigma, r0, nc);
+  HPCGX_CHECK_LAUNCH(c);
+  return symgs_launch<VT, XT, ACC>(c, L, r, x, false);
+}
+
+// vector kernels
+
+__global__ __launch_bounds__(kBlock) void k_waxpby(int64_t n, double alpha,
+                                                   const double *__restrict__ x, double beta,
+                                                   const double *__restrict__ y,
+                                                   double *__restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  w[i] = waxpby1(alpha, x[i], beta, y[i]);
+}
+
+// out = x . y (ComputeDotProduct_ref.cpp:54-64, fixed-order tree)
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *__restrict__ x,
+                                                const double *__restrict__ y, double *partials,
+                                                unsigned *ticket, double *out) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    acc += x[i] * y[i];
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, ticket, tot)) out[0] = tot[0];
+}
+
+// w = alpha x + beta y ; out = w . w   (WAXPBY then DOT of CG.cpp:88-89 / 122-123)
+__global__ __launch_bounds__(kBlock) void k_waxpby_dot(int64_t n, double alpha,
+                                                       const double *__restrict__ x, double beta,
+                                                       const double *__restrict__ y,
+                                                       double *__restrict__ w, double *partials,
+                                                       unsigned *ticket, double *out) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double v = waxpby1(alpha, x[i], beta, y[i]);
+    w[i] = v;
+    acc += v * v;
+  }
+  double v[1] = {acc}, tot[1];
+  if (grid_reduce<1>(v, partials, ticket, tot)) out[0] = tot[0];
+}
+
+int mg_fp64(Ctx *c, int level, const double *r, double *x) {
+  return vcycle<double, double, double, false>(c, level, r, x);
+}
+
+int mg_fp32(Ctx *c, int level, const float *r, float *x) {
+  return vcycle<float, float, float, false>(c, level, r, x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(const float *__restrict__ a, const float *__restrict__ s,
+                                                  float *__restrict__ o, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) o[i] = a[i] * s[i];
+}
+
+// Scaled fp16 V-cycle on level 0: rhat = dinv r, V-cycle with D^-1/2 A D^-1/2,
+// x = dinv xhat.
+int mg_fp16(Ctx *c, const float *r, float *x) {
+  Level &L = c->levels[0];
+  const int64_t np = L.lay.npad;
+  k_scale<<<tiles(np), kBlock, 0, c->stream>>>(r, L.dinv, L.r16, np);
+  HPCGX_CHECK_LAUNCH(c);
+  int rc = vcycle<__half, float, float, true>(c, 0, L.r16, x);
+  if (rc) return rc;
+  k_scale<<<tiles(np), kBlock, 0, c->stream>>>(x, L.dinv, x, np);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_single(const double *__restrict__ a, float *__restrict__ o,
+                                                      int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) o[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_double(const float *__restrict__ a, double *__restrict__ o,
+                                                      int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) o[i] = (double)a[i];
+}
+
+int spmv_fp64(Ctx *c, int level, const double *x, double *y) {
+  const Level &L = c->levels[level];
+  k_spmv<double, double, double><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, x, y, 0,
+                                                                              L.lay.npad);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+int symgs_fp64(Ctx *c, int level, const double *r, double *x) {
+  return symgs_launch<double, double, double>(c, c->levels[level], r, x, false);
+}
+
+}  // namespace hpcgx
+
+using namespace hpcgx;
+
+static int check_level(hpcgx_ctx *c, int level, const char *who) {
+  if (!c) return HPCGX_EINVAL;
+  if (level < 0 || level >= (int)c->levels.size())
+    return set_err(c, HPCGX_EINVAL, std::string(who) + ": no such level");
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(c, HPCGX_EHIP, "hipSetDevice failed");
+  return ensure_ready(c);
+}
+
+extern "C" {
+
+int hpcgx_spmv(hpcgx_ctx *c, int level, const double *x, double *y) {
+  int rc = check_level(c, level, "spmv");
+  if (rc) return rc;
+  if (!x || !y || x == y) return set_err(c, HPCGX_EINVAL, "spmv: x and y must be distinct device vectors");
+  return spmv_fp64(c, level, x, y);
+}
+
+int hpcgx_symgs(hpcgx_ctx *c, int level, const double *r, double *x) {
+  int rc = check_level(c, level, "symgs");
+  if (rc) return rc;
+  if (!r || !x) return set_err(c, HPCGX_EINVAL, "symgs: null vector");
+  return symgs_fp64(c, level, r, x);
+}
+
+int hpcgx_restrict(hpcgx_ctx *c, int level, const double *r, const double *x, double *rc_out) {
+  int rc = check_level(c, level, "restrict");
+  if (rc) return rc;
+  if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "restrict: level has no coarse level");
+  const Level &L = c->levels[level];
+  const int64_t nc = L.lay.count[7];
+  k_restrict<double, double, double, false><<<tiles(nc), kBlock, 0, c->stream>>>(
+      L.cols, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[7], nc);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+int hpcgx_prolong(hpcgx_ctx *c, int level, const double *xc, double *x) {
+  int rc = check_level(c, level, "prolong");
+  if (rc) return rc;
+  if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "prolong: level has no coarse level");
+  const Level &L = c->levels[level];
+  const int64_t nc = L.lay.count[7];
+  k_prolong<double, double, false><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc, L.inj, nullptr,
+                                                                        L.lay.start[7], nc);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+int hpcgx_mg(hpcgx_ctx *c, int level, const double *r, double *x) {
+  int rc = check_level(c, level, "mg");
+  if (rc) return rc;
+  if (!r || !x || r == x) return set_err(c, HPCGX_EINVAL, "mg: r and x must be distinct device vectors");
+  return mg_fp64(c, level, r, x);
+}
+
+int hpcgx_mg_mixed(hpcgx_ctx *c, int precision, const double *r, double *x) {
+  int rc = check_level(c, 0, "mg_mixed");
+  if (rc) return rc;
+  if (!r || !x || r == x) return set_err(c, HPCGX_EINVAL, "mg_mixed: r and x must be distinct device vectors");
+  const int need = precision == HPCGX_PREC_FP32 ? 1 : precision == HPCGX_PREC_FP16 ? 2 : 0;
+  if (!need) return set_err(c, HPCGX_EINVAL, "mg_mixed: precision must be HPCGX_PREC_FP32 or HPCGX_PREC_FP16");
+  if (!(c->prepared & need)) return set_err(c, HPCGX_ESTATE, "mg_mixed: call hpcgx_prepare_precision first");
+  const int64_t np = c->levels[0].lay.npad;
+  k_to_single<<<tiles(np), kBlock, 0, c->stream>>>(r, c->cg_r32, np);
+  HPCGX_CHECK_LAUNCH(c);
+  rc = precision == HPCGX_PREC_FP32 ? mg_fp32(c, 0, c->cg_r32, c->cg_z32) : mg_fp16(c, c->cg_r32, c->cg_z32);
+  if (rc) return rc;
+  k_to_double<<<tiles(np), kBlock, 0, c->stream>>>(c->cg_z32, x, np);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+static int check_n(hpcgx_ctx *c, int64_t n, const char *who) {
+  int rc = check_level(c, 0, who);
+  if (rc) return rc;
+  if (n < 0 || n > c->levels[0].lay.npad) return set_err(c, HPCGX_EINVAL, std::string(who) + ": bad length");
+  return HPCGX_OK;
+}
+
+// Length n refers to the level-0 device vector; n equal to the level's true
+// row count covers the whole padded vector (padding entries are zero).
+static int64_t span(hpcgx_ctx *c, int64_t n) {
+  return n == c->levels[0].lay.n ? c->levels[0].lay.npad : n;
+}
+
+int hpcgx_dot(hpcgx_ctx *c, int64_t n, const double *x, const double *y, double *result) {
+  int rc = check_n(c, n, "dot");
+  if (rc) return rc;
+  if (!x || !y || !result) return set_err(c, HPCGX_EINVAL, "dot: null argument");
+  const int64_t m = span(c, n);
+  k_dot<<<reduce_grid_size(m), kBlock, 0, c->stream>>>(m, x, y, c->partials, c->ticket, c->scal + S_DOT);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipMemcpyAsync(c->hbuf, c->scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  *result = c->hbuf[0];
+  return HPCGX_OK;
+}
+
+int hpcgx_waxpby(hpcgx_ctx *c, int64_t n, double alpha, const double *x, double beta, const double *y,
+                 double *w) {
+  int rc = check_n(c, n, "waxpby");
+  if (rc) return rc;
+  if (!x || !y || !w) return set_err(c, HPCGX_EINVAL, "waxpby: null vector");
+  const int64_t m = span(c, n);
+  k_waxpby<<<tiles(m), kBlock, 0, c->stream>>>(m, alpha, x, beta, y, w);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+int hpcgx_waxpby_dot(hpcgx_ctx *c, int64_t n, double alpha, const double *x, double beta, const double *y,
+                     double *w, double *result) {
+  int rc = check_n(c, n, "waxpby_dot");
+  if (rc) return rc;
+  if (!x || !y || !w || !result) return set_err(c, HPCGX_EINVAL, "waxpby_dot: null argument");
+  const int64_t m = span(c, n);
+  k_waxpby_dot<<<reduce_grid_size(m), kBlock, 0, c->stream>>>(m, alpha, x, beta, y, w, c->partials, c->ticket,
+                                                              c->scal + S_DOT);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipMemcpyAsync(c->hbuf, c->scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  *result = c->hbuf[0];
+  return HPCGX_OK;
+}
+
+}  // extern "C"

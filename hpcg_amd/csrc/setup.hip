@@ -1,0 +1,796 @@
+// Context lifecycle, problem upload / on-device generation, colour layout,
+// lower-precision copies and vector transfers.  This is the OptimizeProblem
+// side of the library (OptimizeProblem.cpp:37-107): colouring, reordering and
+// packing happen once, here; the iteration kernels only read the result.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "kernels.hpp"
+
+namespace hpcgx {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+int set_err(Ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int dalloc(Ctx *c, void **p, size_t bytes, bool zero) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return set_err(c, HPCGX_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  if (zero) {
+    e = hipMemsetAsync(*p, 0, bytes, c->stream);
+    if (e != hipSuccess) return set_err(c, HPCGX_EHIP, "hipMemsetAsync failed");
+  }
+  c->bytes += (int64_t)bytes;
+  return HPCGX_OK;
+}
+
+void dfree(Ctx *c, void *p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  c->bytes -= (int64_t)(bytes ? bytes : 8);
+}
+
+template <class F>
+static void parallel_for(int64_t n, F f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 65536) nt = 1;
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const int64_t lo = t * chunk, hi = std::min<int64_t>(n, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back([=] {
+      for (int64_t i = lo; i < hi; ++i) f(i);
+    });
+  }
+  for (auto &t : th) t.join();
+}
+
+Layout make_layout(int nx, int ny, int nz) {
+  Layout L;
+  L.nx = nx;
+  L.ny = ny;
+  L.nz = nz;
+  L.n = (int64_t)nx * ny * nz;
+  L.start[0] = 0;
+  for (int k = 0; k < kColours; ++k) {
+    const int c = 7 - k;
+    L.nb[k][0] = (nx + 1 - (c & 1)) / 2;
+    L.nb[k][1] = (ny + 1 - ((c >> 1) & 1)) / 2;
+    L.nb[k][2] = (nz + 1 - ((c >> 2) & 1)) / 2;
+    L.count[k] = (int64_t)L.nb[k][0] * L.nb[k][1] * L.nb[k][2];
+    const int64_t padded = (L.count[k] + kSlice - 1) / kSlice * kSlice;
+    L.start[k + 1] = L.start[k] + padded;
+  }
+  L.npad = L.start[kColours];
+  return L;
+}
+
+static LayoutDev to_dev(const Layout &L) {
+  LayoutDev d;
+  d.nx = L.nx;
+  d.ny = L.ny;
+  d.nz = L.nz;
+  for (int k = 0; k <= kColours; ++k) d.start[k] = L.start[k];
+  for (int k = 0; k < kColours; ++k)
+    for (int a = 0; a < 3; ++a) d.nb[k][a] = L.nb[k][a];
+  return d;
+}
+
+int reduce_grid_size(int64_t n) {
+  const int64_t tiles = (n + kBlock - 1) / kBlock;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kReduceBlocks));
+}
+
+int ensure_stage(Ctx *c, int64_t n) {
+  if (c->stage_len >= n) return HPCGX_OK;
+  if (c->stage) dfree(c, c->stage, c->stage_len * sizeof(double));
+  c->stage = nullptr;
+  c->stage_len = 0;
+  int rc = dalloc(c, (void **)&c->stage, n * sizeof(double), false);
+  if (rc) return rc;
+  c->stage_len = n;
+  return HPCGX_OK;
+}
+
+static void free_level(Ctx *c, Level &L) {
+  const size_t m = (size_t)L.lay.npad * kSlots;
+  dfree(c, L.cols, m * sizeof(int));
+  dfree(c, L.vals, m * sizeof(double));
+  dfree(c, L.vals32, m * sizeof(float));
+  dfree(c, L.vals16, m * sizeof(__half));
+  dfree(c, L.dinv, L.lay.npad * sizeof(float));
+  dfree(c, L.sigma, L.lay.count[7] * sizeof(float));
+  dfree(c, L.perm, L.lay.n * sizeof(int));
+  dfree(c, L.iperm, L.lay.npad * sizeof(int));
+  dfree(c, L.inj, L.lay.count[7] * sizeof(int));
+  dfree(c, L.r, L.lay.npad * sizeof(double));
+  dfree(c, L.x, L.lay.npad * sizeof(double));
+  dfree(c, L.r32, L.lay.npad * sizeof(float));
+  dfree(c, L.x32, L.lay.npad * sizeof(float));
+  dfree(c, L.r16, L.lay.npad * sizeof(float));
+  L = Level();
+}
+
+static int alloc_level(Ctx *c, Level &L) {
+  const size_t m = (size_t)L.lay.npad * kSlots;
+  int rc;
+  if ((rc = dalloc(c, (void **)&L.cols, m * sizeof(int), false))) return rc;
+  if ((rc = dalloc(c, (void **)&L.vals, m * sizeof(double), false))) return rc;
+  if ((rc = dalloc(c, (void **)&L.perm, L.lay.n * sizeof(int), false))) return rc;
+  if ((rc = dalloc(c, (void **)&L.iperm, L.lay.npad * sizeof(int), false))) return rc;
+  return HPCGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// natural order <-> colour order
+
+__global__ __launch_bounds__(kBlock) void k_gather_natural(double *__restrict__ dst,
+                                                           const double *__restrict__ src,
+                                                           const int *__restrict__ iperm,
+                                                           int64_t npad) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npad) return;
+  const int i = iperm[p];
+  dst[p] = i >= 0 ? src[i] : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter_natural(double *__restrict__ dst,
+                                                            const double *__restrict__ src,
+                                                            const int *__restrict__ perm,
+                                                            int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  dst[i] = src[perm[i]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double *__restrict__ v, const int *__restrict__ iperm,
+                                                 double value, int64_t npad) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npad) return;
+  v[p] = iperm[p] >= 0 ? value : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// on-device generation: GenerateProblem_ref.cpp:131-183 in colour order
+
+__global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, int *__restrict__ cols,
+                                                     double *__restrict__ vals,
+                                                     int *__restrict__ iperm, int *__restrict__ perm,
+                                                     const double *__restrict__ kappa) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npad) return;
+  int k = 0;
+  while (k < 7 && p >= L.start[k + 1]) ++k;
+  const int64_t idx = p - L.start[k];
+  const int nbx = L.nb[k][0], nby = L.nb[k][1], nbz = L.nb[k][2];
+  const size_t b = ell_base(p);
+  if (idx >= (int64_t)nbx * nby * nbz) {  // padding: identity row
+    iperm[p] = -1;
+    for (int s = 0; s < kSlots; ++s) {
+      cols[b + s * kSlice] = (int)p;
+      vals[b + s * kSlice] = s == kDiagSlot ? 1.0 : 0.0;
+    }
+    return;
+  }
+  const int c = 7 - k;
+  const int ix = 2 * (int)(idx % nbx) + (c & 1);
+  const int iy = 2 * (int)((idx / nbx) % nby) + ((c >> 1) & 1);
+  const int iz = 2 * (int)(idx / ((int64_t)nbx * nby)) + ((c >> 2) & 1);
+  const int64_t i = ((int64_t)iz * L.ny + iy) * L.nx + ix;
+  iperm[p] = (int)i;
+  perm[i] = (int)p;
+  const double ki = kappa ? kappa[i] : 1.0;
+  int nn = 0;
+  double dsum = 0.0;
+  for (int s = 0; s < kSlots; ++s) {
+    const int jx = ix + s % 3 - 1, jy = iy + (s / 3) % 3 - 1, jz = iz + s / 9 - 1;
+    const bool in = jx >= 0 && jx < L.nx && jy >= 0 && jy < L.ny && jz >= 0 && jz < L.nz;
+    if (!in) {
+      cols[b + s * kSlice] = (int)p;
+      vals[b + s * kSlice] = 0.0;
+      continue;
+    }
+    cols[b + s * kSlice] = (int)perm_of(L, jx, jy, jz);
+    if (s == kDiagSlot) continue;
+    ++nn;
+    if (kappa) {
+      const double kj = kappa[((int64_t)jz * L.ny + jy) * L.nx + jx];
+      vals[b + s * kSlice] = -((ki + kj) * 0.5);
+      dsum += (ki + kj) * 0.5;
+    } else {
+      vals[b + s * kSlice] = -1.0;
+    }
+  }
+  vals[b + kDiagSlot * kSlice] = kappa ? dsum + (double)(26 - nn) * ki : 26.0;
+}
+
+// coarse kappa by injection (GenerateCoarseProblem.cpp:71-82 applied to kappa)
+__global__ void k_inject_kappa(const double *__restrict__ kf, double *__restrict__ kc, int nxf, int nyf,
+                               int ncx, int ncy, int64_t nc) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nc) return;
+  const int a = (int)(t % ncx), b = (int)((t / ncx) % ncy), cc = (int)(t / ((int64_t)ncx * ncy));
+  kc[t] = kf[((int64_t)(2 * cc) * nyf + 2 * b) * nxf + 2 * a];
+}
+
+// inj for generated levels: the t-th injected fine row is coarse natural row t
+__global__ void k_inj_generated(LayoutDev Lc, int *__restrict__ inj, int64_t nc) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nc) return;
+  const int a = (int)(t % Lc.nx), b = (int)((t / Lc.nx) % Lc.ny), cc = (int)(t / ((int64_t)Lc.nx * Lc.ny));
+  inj[t] = (int)perm_of(Lc, a, b, cc);
+}
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// upload from the reference's structures
+
+template <class RowAcc>
+static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, const int *f2c) {
+  if (level < 0 || level >= kMaxLevels || nx < 1 || ny < 1 || nz < 1)
+    return set_err(c, HPCGX_EINVAL, "level_upload: bad level or grid");
+  if ((int)c->levels.size() < level) return set_err(c, HPCGX_ESTATE, "levels must be uploaded fine to coarse");
+  if ((int)c->levels.size() > level) {
+    for (int l = (int)c->levels.size() - 1; l >= level; --l) free_level(c, c->levels[l]);
+    c->levels.resize(level);
+  }
+  c->ready = false;
+  c->prepared = 0;
+  c->levels.emplace_back();
+  Level &L = c->levels.back();
+  L.lay = make_layout(nx, ny, nz);
+  L.ldev = to_dev(L.lay);
+  const Layout &lay = L.lay;
+  const int64_t n = lay.n, npad = lay.npad;
+  if (npad >= ((int64_t)1 << 31)) return set_err(c, HPCGX_EINVAL, "level_upload: too many rows for 32-bit columns");
+  L.perm_h.assign(n, 0);
+  L.iperm_h.assign(npad, -1);
+  for (int k = 0; k < kColours; ++k) {
+    const int cl = 7 - k;
+    for (int64_t idx = 0; idx < lay.count[k]; ++idx) {
+      const int ix = 2 * (int)(idx % lay.nb[k][0]) + (cl & 1);
+      const int iy = 2 * (int)((idx / lay.nb[k][0]) % lay.nb[k][1]) + ((cl >> 1) & 1);
+      const int iz = 2 * (int)(idx / ((int64_t)lay.nb[k][0] * lay.nb[k][1])) + ((cl >> 2) & 1);
+      const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
+      L.perm_h[i] = (int)(lay.start[k] + idx);
+      L.iperm_h[lay.start[k] + idx] = (int)i;
+    }
+  }
+  const size_t m = (size_t)npad * kSlots;
+  std::vector<int> hc(m);
+  std::vector<double> hv(m);
+  std::atomic<int> bad(0);
+  std::atomic<int64_t> nnz(0);
+  parallel_for(npad, [&](int64_t p) {
+    const size_t b = (size_t)(p >> 6) * (kSlots * kSlice) + (size_t)(p & 63);
+    const int i = L.iperm_h[p];
+    for (int s = 0; s < kSlots; ++s) {
+      hc[b + s * kSlice] = (int)p;
+      hv[b + s * kSlice] = (i < 0 && s == kDiagSlot) ? 1.0 : 0.0;
+    }
+    if (i < 0) return;
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
+    const int cnt = rows.nnz(i);
+    const int *rc = rows.cols(i);
+    const double *rv = rows.vals(i);
+    if (cnt < 1 || cnt > kSlots) { bad = 1; return; }
+    int last = -1;
+    bool diag = false;
+    for (int j = 0; j < cnt; ++j) {
+      const int col = rc[j];
+      if (col < 0 || col >= n) { bad = 1; return; }
+      const int dx = col % nx - ix, dy = (col / nx) % ny - iy, dz = col / (nx * ny) - iz;
+      if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) { bad = 1; return; }
+      const int s = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+      if (s <= last) { bad = 1; return; }
+      last = s;
+      if (s == kDiagSlot) diag = true;
+      hc[b + s * kSlice] = L.perm_h[col];
+      hv[b + s * kSlice] = rv[j];
+    }
+    if (!diag) { bad = 1; return; }
+    nnz += cnt;
+  });
+  if (bad) return set_err(c, HPCGX_ENOTSTENCIL, "level_upload: matrix is not a 27-point stencil with ascending columns on the stated grid");
+  L.nnz = nnz;
+  int rc;
+  if ((rc = alloc_level(c, L))) return rc;
+  HPCGX_HIP(c, hipMemcpy(L.cols, hc.data(), m * sizeof(int), hipMemcpyHostToDevice));
+  HPCGX_HIP(c, hipMemcpy(L.vals, hv.data(), m * sizeof(double), hipMemcpyHostToDevice));
+  HPCGX_HIP(c, hipMemcpy(L.perm, L.perm_h.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  HPCGX_HIP(c, hipMemcpy(L.iperm, L.iperm_h.data(), npad * sizeof(int), hipMemcpyHostToDevice));
+  if (f2c) {
+    if (nx % 2 || ny % 2 || nz % 2) return set_err(c, HPCGX_EINVAL, "level_upload: f2c given for a grid that cannot be halved");
+    const int64_t nc = (int64_t)(nx / 2) * (ny / 2) * (nz / 2);
+    L.f2c_h.assign(f2c, f2c + nc);
+  }
+  return HPCGX_OK;
+}
+
+struct PackedRows {
+  const uint8_t *nr;
+  const int *c;
+  const double *v;
+  int nnz(int64_t i) const { return nr[i]; }
+  const int *cols(int64_t i) const { return c + i * kSlots; }
+  const double *vals(int64_t i) const { return v + i * kSlots; }
+};
+
+struct PointerRows {
+  const char *nr;
+  const int *const *c;
+  const double *const *v;
+  int nnz(int64_t i) const { return (int)(unsigned char)nr[i]; }
+  const int *cols(int64_t i) const { return c[i]; }
+  const double *vals(int64_t i) const { return v[i]; }
+};
+
+// Builds what the V-cycle needs once every level is present: the injection
+// maps, the coarse work vectors and the solver buffers.
+int ensure_ready(Ctx *c) {
+  if (c->ready) return HPCGX_OK;
+  if (c->levels.empty()) return set_err(c, HPCGX_ESTATE, "no problem: upload or generate levels first");
+  const int nl = (int)c->levels.size();
+  int rc;
+  for (int l = 0; l + 1 < nl; ++l) {
+    Level &F = c->levels[l], &Cc = c->levels[l + 1];
+    const int64_t nc = Cc.lay.n;
+    if (F.inj) continue;
+    if (F.lay.count[7] != nc || F.lay.nx != 2 * Cc.lay.nx || F.lay.ny != 2 * Cc.lay.ny ||
+        F.lay.nz != 2 * Cc.lay.nz)
+      return set_err(c, HPCGX_ESTATE, "level " + std::to_string(l + 1) + " is not the 2x coarsening of level " + std::to_string(l));
+    if ((int64_t)F.f2c_h.size() != nc)
+      return set_err(c, HPCGX_ESTATE, "level " + std::to_string(l) + " has no f2c operator");
+    std::vector<int> inv(F.lay.npad, -1);
+    for (int64_t k = 0; k < nc; ++k) {
+      const int f = F.f2c_h[k];
+      if (f < 0 || f >= F.lay.n) return set_err(c, HPCGX_EINVAL, "f2c out of range");
+      const int pf = F.perm_h[f];
+      if (pf < F.lay.start[7] || pf >= F.lay.start[7] + F.lay.count[7] || inv[pf] >= 0)
+        return set_err(c, HPCGX_EINVAL, "f2c must map coarse rows one-to-one onto fine rows with even coordinates");
+      inv[pf] = (int)k;
+    }
+    std::vector<int> inj(nc);
+    for (int64_t t = 0; t < nc; ++t) inj[t] = Cc.perm_h[inv[F.lay.start[7] + t]];
+    if ((rc = dalloc(c, (void **)&F.inj, nc * sizeof(int), false))) return rc;
+    HPCGX_HIP(c, hipMemcpy(F.inj, inj.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+  }
+  for (int l = 1; l < nl; ++l) {
+    Level &L = c->levels[l];
+    if (!L.r && (rc = dalloc(c, (void **)&L.r, L.lay.npad * sizeof(double)))) return rc;
+    if (!L.x && (rc = dalloc(c, (void **)&L.x, L.lay.npad * sizeof(double)))) return rc;
+  }
+  const int64_t np = c->levels[0].lay.npad;
+  if (!c->cg_r) {
+    if ((rc = dalloc(c, (void **)&c->cg_r, np * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void **)&c->cg_z, np * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void **)&c->cg_p, np * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void **)&c->cg_Ap, np * sizeof(double)))) return rc;
+  }
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  c->ready = true;
+  return HPCGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// lower-precision copies
+
+__global__ void k_to_f32(const double *__restrict__ v, float *__restrict__ o, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < m) o[i] = (float)v[i];
+}
+
+// dinv = 1/sqrt(a_ii) per row (padding: a_ii = 1)
+__global__ void k_dinv(const double *__restrict__ vals, float *__restrict__ dinv, int64_t npad) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npad) return;
+  dinv[p] = (float)(1.0 / sqrt(vals[ell_base(p) + kDiagSlot * kSlice]));
+}
+
+// ahat_ij = a_ij * dinv_i * dinv_j in fp64, rounded to fp32 then fp16; diagonal 1
+__global__ void k_to_f16_scaled(const int *__restrict__ cols, const double *__restrict__ vals,
+                                __half *__restrict__ v16, int64_t npad) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npad) return;
+  const size_t b = ell_base(p);
+  const double di = 1.0 / sqrt(vals[b + kDiagSlot * kSlice]);
+  for (int s = 0; s < kSlots; ++s) {
+    double a;
+    if (s == kDiagSlot) {
+      a = 1.0;
+    } else {
+      const int cj = cols[b + s * kSlice];
+      const double dj = 1.0 / sqrt(vals[ell_base(cj) + kDiagSlot * kSlice]);
+      a = vals[b + s * kSlice] * di * dj;
+    }
+    v16[b + s * kSlice] = __float2half_rn((float)a);
+  }
+}
+
+// sigma_t = sqrt(d_f / d_c) for the t-th injected row
+__global__ void k_sigma(const double *__restrict__ fvals, const double *__restrict__ cvals,
+                        const int *__restrict__ inj, float *__restrict__ sigma, int64_t row0,
+                        int64_t nc) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nc) return;
+  const double df = fvals[ell_base(row0 + t) + kDiagSlot * kSlice];
+  const double dc = cvals[ell_base(inj[t]) + kDiagSlot * kSlice];
+  sigma[t] = (float)sqrt(df / dc);
+}
+
+static int build_precision(Ctx *c, int prec) {
+  int rc = ensure_ready(c);
+  if (rc) return rc;
+  const int nl = (int)c->levels.size();
+  for (int l = 0; l < nl; ++l) {
+    Level &L = c->levels[l];
+    const int64_t m = L.lay.npad * kSlots;
+    if (prec == HPCGX_PREC_FP32) {
+      if (!L.vals32 && (rc = dalloc(c, (void **)&L.vals32, m * sizeof(float), false))) return rc;
+      k_to_f32<<<nblk(m), kBlock, 0, c->stream>>>(L.vals, L.vals32, m);
+      HPCGX_CHECK_LAUNCH(c);
+    } else {
+      if (!L.vals16 && (rc = dalloc(c, (void **)&L.vals16, m * sizeof(__half), false))) return rc;
+      if (!L.dinv && (rc = dalloc(c, (void **)&L.dinv, L.lay.npad * sizeof(float), false))) return rc;
+      k_to_f16_scaled<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, L.vals16, L.lay.npad);
+      k_dinv<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.vals, L.dinv, L.lay.npad);
+      HPCGX_CHECK_LAUNCH(c);
+    }
+    if (!L.r32 && l > 0 && (rc = dalloc(c, (void **)&L.r32, L.lay.npad * sizeof(float)))) return rc;
+    if (!L.x32 && l > 0 && (rc = dalloc(c, (void **)&L.x32, L.lay.npad * sizeof(float)))) return rc;
+  }
+  if (prec == HPCGX_PREC_FP16) {
+    for (int l = 0; l + 1 < nl; ++l) {
+      Level &F = c->levels[l];
+      const int64_t nc = F.lay.count[7];
+      if (!F.sigma && (rc = dalloc(c, (void **)&F.sigma, nc * sizeof(float), false))) return rc;
+      k_sigma<<<nblk(nc), kBlock, 0, c->stream>>>(F.vals, c->levels[l + 1].vals, F.inj, F.sigma,
+                                                  F.lay.start[7], nc);
+      HPCGX_CHECK_LAUNCH(c);
+    }
+    Level &L0 = c->levels[0];
+    if (!L0.r16 && (rc = dalloc(c, (void **)&L0.r16, L0.lay.npad * sizeof(float)))) return rc;
+  }
+  const int64_t np = c->levels[0].lay.npad;
+  if (!c->cg_r32 && (rc = dalloc(c, (void **)&c->cg_r32, np * sizeof(float)))) return rc;
+  if (!c->cg_z32 && (rc = dalloc(c, (void **)&c->cg_z32, np * sizeof(float)))) return rc;
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  c->prepared |= (prec == HPCGX_PREC_FP32) ? 1 : 2;
+  return HPCGX_OK;
+}
+
+// diagonal scatter / gather (natural order staging <-> colour-order ELL)
+__global__ void k_set_diag(double *__restrict__ vals, const int *__restrict__ perm,
+                           const double *__restrict__ d, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) vals[ell_base(perm[i]) + kDiagSlot * kSlice] = d[i];
+}
+__global__ void k_get_diag(const double *__restrict__ vals, const int *__restrict__ perm,
+                           double *__restrict__ d, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) d[i] = vals[ell_base(perm[i]) + kDiagSlot * kSlice];
+}
+
+}  // namespace hpcgx
+
+using namespace hpcgx;
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int hpcgx_create(int device, hpcgx_ctx **out) {
+  if (!out) return HPCGX_EINVAL;
+  *out = nullptr;
+  hpcgx_ctx *c = new hpcgx_ctx();
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete c;
+    return HPCGX_EHIP;
+  }
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return HPCGX_EHIP;
+  }
+  c->stream = c->own_stream;
+  int rc;
+  if ((rc = dalloc(c, (void **)&c->partials, 4 * kReduceBlocks * sizeof(double))) ||
+      (rc = dalloc(c, (void **)&c->ticket, 64 * sizeof(unsigned))) ||
+      (rc = dalloc(c, (void **)&c->scal, S_COUNT * sizeof(double)))) {
+    hpcgx_destroy(c);
+    return rc;
+  }
+  if (hipHostMalloc((void **)&c->hbuf, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    hpcgx_destroy(c);
+    return HPCGX_ENOMEM;
+  }
+  for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    hpcgx_destroy(c);
+    return HPCGX_EHIP;
+  }
+  *out = c;
+  return HPCGX_OK;
+}
+
+int hpcgx_destroy(hpcgx_ctx *c) {
+  if (!c) return HPCGX_EINVAL;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &L : c->levels) free_level(c, L);
+  for (void *p : c->user_vecs) (void)hipFree(p);
+  (void)hipFree(c->cg_r);
+  (void)hipFree(c->cg_z);
+  (void)hipFree(c->cg_p);
+  (void)hipFree(c->cg_Ap);
+  (void)hipFree(c->cg_r32);
+  (void)hipFree(c->cg_z32);
+  (void)hipFree(c->stage);
+  (void)hipFree(c->partials);
+  (void)hipFree(c->ticket);
+  (void)hipFree(c->scal);
+  (void)hipFree(c->hist);
+  if (c->hbuf) (void)hipHostFree(c->hbuf);
+  for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return HPCGX_OK;
+}
+
+const char *hpcgx_error_string(const hpcgx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int hpcgx_set_stream(hpcgx_ctx *c, void *stream) {
+  if (!c) return HPCGX_EINVAL;
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return HPCGX_OK;
+}
+
+int hpcgx_sync(hpcgx_ctx *c) {
+  if (!c) return HPCGX_EINVAL;
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  return HPCGX_OK;
+}
+
+int hpcgx_level_upload_packed(hpcgx_ctx *c, int level, int nx, int ny, int nz, const uint8_t *nnz_row,
+                              const int *cols, const double *vals, const int *f2c) {
+  if (!c || !nnz_row || !cols || !vals) return set_err(c, HPCGX_EINVAL, "level_upload_packed: null argument");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  return upload_level(c, level, nx, ny, nz, PackedRows{nnz_row, cols, vals}, f2c);
+}
+
+int hpcgx_level_upload(hpcgx_ctx *c, int level, int nx, int ny, int nz, const char *nonzerosInRow,
+                       const int *const *mtxIndL, const double *const *matrixValues, const int *f2c) {
+  if (!c || !nonzerosInRow || !mtxIndL || !matrixValues) return set_err(c, HPCGX_EINVAL, "level_upload: null argument");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  return upload_level(c, level, nx, ny, nz, PointerRows{nonzerosInRow, mtxIndL, matrixValues}, f2c);
+}
+
+int hpcgx_problem_generate(hpcgx_ctx *c, int nx, int ny, int nz, int levels, const double *kappa) {
+  if (!c || nx < 1 || ny < 1 || nz < 1 || levels < 1 || levels > kMaxLevels)
+    return set_err(c, HPCGX_EINVAL, "problem_generate: bad grid or level count");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  for (int l = 1; l < levels; ++l)
+    if (((nx >> (l - 1)) & 1) || ((ny >> (l - 1)) & 1) || ((nz >> (l - 1)) & 1))
+      return set_err(c, HPCGX_EINVAL, "problem_generate: grid cannot be halved that many times");
+  for (int l = (int)c->levels.size() - 1; l >= 0; --l) free_level(c, c->levels[l]);
+  c->levels.clear();
+  c->ready = false;
+  c->prepared = 0;
+  double *kdev = nullptr, *kprev = nullptr;
+  int rc;
+  int64_t kprev_n = 0;
+  for (int l = 0; l < levels; ++l) {
+    c->levels.emplace_back();
+    Level &L = c->levels.back();
+    L.lay = make_layout(nx >> l, ny >> l, nz >> l);
+    L.ldev = to_dev(L.lay);
+    L.kappa = kappa != nullptr;
+    if (L.lay.npad >= ((int64_t)1 << 31)) return set_err(c, HPCGX_EINVAL, "problem_generate: too many rows for 32-bit columns");
+    if ((rc = alloc_level(c, L))) return rc;
+    if (kappa) {
+      if ((rc = dalloc(c, (void **)&kdev, L.lay.n * sizeof(double), false))) return rc;
+      if (l == 0) {
+        HPCGX_HIP(c, hipMemcpy(kdev, kappa, L.lay.n * sizeof(double), hipMemcpyHostToDevice));
+      } else {
+        const Layout &F = c->levels[l - 1].lay;
+        k_inject_kappa<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(kprev, kdev, F.nx, F.ny, L.lay.nx,
+                                                                L.lay.ny, L.lay.n);
+        HPCGX_CHECK_LAUNCH(c);
+      }
+    }
+    k_generate<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.ldev, L.lay.npad, L.cols, L.vals, L.iperm,
+                                                          L.perm, kdev);
+    HPCGX_CHECK_LAUNCH(c);
+    int64_t nnz = 1;
+    for (const int d : {L.lay.nx, L.lay.ny, L.lay.nz}) nnz *= (int64_t)(3 * d - 2);
+    L.nnz = nnz;
+    if (kprev) {
+      HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+      dfree(c, kprev, kprev_n * sizeof(double));
+    }
+    kprev = kdev;
+    kprev_n = L.lay.n;
+    kdev = nullptr;
+  }
+  if (kprev) {
+    HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c, kprev, kprev_n * sizeof(double));
+  }
+  for (int l = 0; l + 1 < levels; ++l) {
+    Level &F = c->levels[l];
+    const Level &Cc = c->levels[l + 1];
+    if ((rc = dalloc(c, (void **)&F.inj, Cc.lay.n * sizeof(int), false))) return rc;
+    k_inj_generated<<<nblk(Cc.lay.n), kBlock, 0, c->stream>>>(Cc.ldev, F.inj, Cc.lay.n);
+    HPCGX_CHECK_LAUNCH(c);
+  }
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  return ensure_ready(c);
+}
+
+int hpcgx_level_info(const hpcgx_ctx *c, int level, int64_t *out) {
+  if (!c || !out || level < 0 || level >= (int)c->levels.size()) return HPCGX_EINVAL;
+  const Level &L = c->levels[level];
+  out[0] = L.lay.n;
+  out[1] = L.lay.npad;
+  out[2] = L.nnz;
+  out[3] = L.lay.nx;
+  out[4] = L.lay.ny;
+  out[5] = L.lay.nz;
+  out[6] = L.lay.npad * kSlots;
+  out[7] = (int64_t)c->levels.size();
+  return HPCGX_OK;
+}
+
+int hpcgx_level_set_diagonal(hpcgx_ctx *c, int level, const double *diag) {
+  if (!c || !diag || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "set_diagonal: bad argument");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  Level &L = c->levels[level];
+  int rc = ensure_stage(c, L.lay.n);
+  if (rc) return rc;
+  HPCGX_HIP(c, hipMemcpyAsync(c->stage, diag, L.lay.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_set_diag<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(L.vals, L.perm, c->stage, L.lay.n);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  const int prepared = c->prepared;
+  if (prepared & 1) (void)build_precision(c, HPCGX_PREC_FP32);
+  if (prepared & 2) (void)build_precision(c, HPCGX_PREC_FP16);
+  return HPCGX_OK;
+}
+
+int hpcgx_level_get_diagonal(hpcgx_ctx *c, int level, double *diag) {
+  if (!c || !diag || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "get_diagonal: bad argument");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  Level &L = c->levels[level];
+  int rc = ensure_stage(c, L.lay.n);
+  if (rc) return rc;
+  k_get_diag<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(L.vals, L.perm, c->stage, L.lay.n);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipMemcpyAsync(diag, c->stage, L.lay.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  return HPCGX_OK;
+}
+
+int hpcgx_prepare_precision(hpcgx_ctx *c, int precision) {
+  if (!c || (precision != HPCGX_PREC_FP32 && precision != HPCGX_PREC_FP16 && precision != HPCGX_PREC_FP64))
+    return set_err(c, HPCGX_EINVAL, "prepare_precision: bad precision");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  if (precision == HPCGX_PREC_FP64) return ensure_ready(c);
+  return build_precision(c, precision);
+}
+
+double hpcgx_memory_bytes(const hpcgx_ctx *c) { return c ? (double)c->bytes : 0.0; }
+
+int hpcgx_vec_alloc(hpcgx_ctx *c, int level, double **dvec) {
+  if (!c || !dvec || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_alloc: bad argument");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  int rc = dalloc(c, (void **)dvec, c->levels[level].lay.npad * sizeof(double));
+  if (rc) return rc;
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  c->user_vecs.push_back(*dvec);
+  return HPCGX_OK;
+}
+
+int hpcgx_vec_free(hpcgx_ctx *c, double *dvec) {
+  if (!c || !dvec) return HPCGX_EINVAL;
+  auto it = std::find(c->user_vecs.begin(), c->user_vecs.end(), (void *)dvec);
+  if (it == c->user_vecs.end()) return set_err(c, HPCGX_EINVAL, "vec_free: not a vector of this context");
+  c->user_vecs.erase(it);
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  HPCGX_HIP(c, hipFree(dvec));
+  return HPCGX_OK;
+}
+
+int hpcgx_vec_upload(hpcgx_ctx *c, int level, double *dvec, const double *host, int64_t n) {
+  if (!c || !dvec || !host || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_upload: bad argument");
+  Level &L = c->levels[level];
+  if (n != L.lay.n) return set_err(c, HPCGX_EINVAL, "vec_upload: length must equal the level's rows");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  int rc = ensure_stage(c, n);
+  if (rc) return rc;
+  HPCGX_HIP(c, hipMemcpyAsync(c->stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_gather_natural<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(dvec, c->stage, L.iperm, L.lay.npad);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  return HPCGX_OK;
+}
+
+int hpcgx_vec_download(hpcgx_ctx *c, int level, const double *dvec, double *host, int64_t n) {
+  if (!c || !dvec || !host || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_download: bad argument");
+  Level &L = c->levels[level];
+  if (n != L.lay.n) return set_err(c, HPCGX_EINVAL, "vec_download: length must equal the level's rows");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  int rc = ensure_stage(c, n);
+  if (rc) return rc;
+  k_scatter_natural<<<nblk(n), kBlock, 0, c->stream>>>(c->stage, dvec, L.perm, n);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipMemcpyAsync(host, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  return HPCGX_OK;
+}
+
+int hpcgx_vec_fill(hpcgx_ctx *c, int level, double *dvec, double value) {
+  if (!c || !dvec || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_fill: bad argument");
+  Level &L = c->levels[level];
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  k_fill<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(dvec, L.iperm, value, L.lay.npad);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+// Test / inspection: the level's matrix in natural row order, packed
+// [n*27] with ascending columns (unused slots: col -1, value 0).
+int hpcgx_level_export_packed(hpcgx_ctx *c, int level, uint8_t *nnz_row, int *cols, double *vals) {
+  if (!c || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "level_export: bad argument");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  Level &L = c->levels[level];
+  const size_t m = (size_t)L.lay.npad * kSlots;
+  std::vector<int> hc(m), ip(L.lay.npad);
+  std::vector<double> hv(m);
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  HPCGX_HIP(c, hipMemcpy(hc.data(), L.cols, m * sizeof(int), hipMemcpyDeviceToHost));
+  HPCGX_HIP(c, hipMemcpy(hv.data(), L.vals, m * sizeof(double), hipMemcpyDeviceToHost));
+  HPCGX_HIP(c, hipMemcpy(ip.data(), L.iperm, L.lay.npad * sizeof(int), hipMemcpyDeviceToHost));
+  const int nx = L.lay.nx, ny = L.lay.ny, nz = L.lay.nz;
+  for (int64_t p = 0; p < L.lay.npad; ++p) {
+    const int i = ip[p];
+    if (i < 0) continue;
+    const size_t b = (size_t)(p >> 6) * (kSlots * kSlice) + (size_t)(p & 63);
+    const int ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
+    int k = 0;
+    for (int s = 0; s < kSlots; ++s) {
+      const int jx = ix + s % 3 - 1, jy = iy + (s / 3) % 3 - 1, jz = iz + s / 9 - 1;
+      if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) continue;
+      const int pc = hc[b + s * kSlice];
+      cols[(int64_t)i * kSlots + k] = pc >= 0 && pc < L.lay.npad ? ip[pc] : -2;
+      vals[(int64_t)i * kSlots + k] = hv[b + s * kSlice];
+      ++k;
+    }
+    nnz_row[i] = (uint8_t)k;
+    for (; k < kSlots; ++k) {
+      cols[(int64_t)i * kSlots + k] = -1;
+      vals[(int64_t)i * kSlots + k] = 0.0;
+    }
+  }
+  return HPCGX_OK;
+}
+
+}  // extern "C"

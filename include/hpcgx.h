@@ -1,0 +1,176 @@
+/*
+ * hpcgx.h -- C ABI of the MI355X-native HPCG PCG iteration phase.
+ *
+ * One shared library (hpcg_amd/libhpcgx.so) exports these symbols with plain
+ * C types: pointers, sizes, int64_t, double.  Host buffers belong to the
+ * caller; device buffers made by hpcgx_vec_alloc and all matrix storage
+ * belong to the context.  Every call returns HPCGX_OK (0) or a negative
+ * code; hpcgx_error_string() describes the last failure of a context.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the HPCG 3.1 reference's src/ directory).  The C++ hooks in
+ * hpcg_amd/dropin/ implement the reference signatures on top of them.
+ *
+ * Device vectors live in the context's row order: rows grouped by colour
+ * (8 colours, the parity classes of the grid coordinates) and padded to 64
+ * rows per colour.  hpcgx_vec_upload / hpcgx_vec_download convert from/to
+ * the reference's natural row order.  Padding entries are always zero.
+ *
+ * Calls are ordered on the context's stream (hpcgx_set_stream); calls that
+ * return a scalar to the host (dot, cg) synchronise that stream.  A context
+ * is used by one host thread at a time.
+ */
+#ifndef HPCGX_H
+#define HPCGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPCGX_OK 0
+#define HPCGX_EINVAL (-1)      /* bad argument */
+#define HPCGX_ENOTSTENCIL (-2) /* matrix is not a 27-point stencil on the stated grid */
+#define HPCGX_EHIP (-3)        /* a HIP runtime call failed */
+#define HPCGX_ENOMEM (-4)      /* device or host allocation failed */
+#define HPCGX_ESTATE (-5)      /* call out of order (e.g. level missing) */
+
+/* storage precision of a multigrid copy */
+#define HPCGX_PREC_FP64 0
+#define HPCGX_PREC_FP32 1
+#define HPCGX_PREC_FP16 2 /* values D^-1/2 A D^-1/2 in fp16, fp32 arithmetic */
+
+/* hpcgx_cg variants */
+#define HPCGX_CG_FP64 0       /* CG.cpp recurrence, fp64 MG */
+#define HPCGX_CG_MIXED_FP32 1 /* flexible PCG, fp64 outer, fp32 MG */
+#define HPCGX_CG_MIXED_FP16 2 /* flexible PCG, fp64 outer, scaled fp16 MG */
+
+typedef struct hpcgx_ctx hpcgx_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Opens a context on a HIP device.  Holds the device state that the
+ * reference keeps behind SparseMatrix::optimizationData
+ * (SparseMatrix.hpp:58-64, OptimizeProblem.cpp:37). */
+int hpcgx_create(int device, hpcgx_ctx **out);
+/* Frees every device buffer of the context (the reference's DeleteMatrix,
+ * SparseMatrix.hpp:151-183, for the device copies). */
+int hpcgx_destroy(hpcgx_ctx *ctx);
+const char *hpcgx_error_string(const hpcgx_ctx *ctx);
+/* Orders all later work on `stream` (a hipStream_t; NULL = the context's own). */
+int hpcgx_set_stream(hpcgx_ctx *ctx, void *stream);
+int hpcgx_sync(hpcgx_ctx *ctx);
+
+/* ---- problem (OptimizeProblem.cpp:37-100) ----------------------------- */
+
+/* Uploads one level in the reference's row-pointer form (SparseMatrix.hpp:47-51:
+ * nonzerosInRow, mtxIndL, matrixValues) of an nx*ny*nz grid, levels from fine
+ * (0) to coarse.  f2c: the level's f2cOperator (MGData.hpp:31), length = rows
+ * of level+1, or NULL on the coarsest level.  Colours, reorders and packs the
+ * rows into 27-slot sliced ELL on the device. */
+int hpcgx_level_upload(hpcgx_ctx *ctx, int level, int nx, int ny, int nz, const char *nonzerosInRow,
+                       const int *const *mtxIndL, const double *const *matrixValues, const int *f2c);
+/* Same, from packed arrays: row i's entries at cols[i*27 + j], vals[i*27 + j]
+ * for j < nnz_row[i], ascending column. */
+int hpcgx_level_upload_packed(hpcgx_ctx *ctx, int level, int nx, int ny, int nz,
+                              const uint8_t *nnz_row, const int *cols, const double *vals,
+                              const int *f2c);
+/* Generates the whole hierarchy on the device from the closed forms of
+ * GenerateProblem_ref.cpp:131-183 and GenerateCoarseProblem.cpp:71-82
+ * (replaces the host generation for large grids).  kappa: NULL for the HPCG
+ * matrix, else the per-cell diffusivity of the fine grid in natural order
+ * (host pointer, nx*ny*nz) for the variable-coefficient operator
+ * a_ij = -(k_i+k_j)/2, a_ii = sum_j (k_i+k_j)/2 + (26-n_i) k_i; coarse
+ * levels take kappa by injection. */
+int hpcgx_problem_generate(hpcgx_ctx *ctx, int nx, int ny, int nz, int levels, const double *kappa);
+/* out[0..7]: rows, padded rows, nonzeros, nx, ny, nz, stored slots, levels */
+int hpcgx_level_info(const hpcgx_ctx *ctx, int level, int64_t *out);
+/* Replaces the diagonal of a level (ReplaceMatrixDiagonal, SparseMatrix.hpp:139-145);
+ * diag is host, natural order.  Lower-precision copies are rebuilt. */
+int hpcgx_level_set_diagonal(hpcgx_ctx *ctx, int level, const double *diag);
+/* Reads the diagonal back (CopyMatrixDiagonal, SparseMatrix.hpp:126-132). */
+int hpcgx_level_get_diagonal(hpcgx_ctx *ctx, int level, double *diag);
+/* Reads a level's device matrix back in the reference's natural row order,
+ * packed like hpcgx_level_upload_packed (unused slots: col -1, value 0); the
+ * inverse of the upload, for inspection and tests (SparseMatrix.hpp:47-51). */
+int hpcgx_level_export_packed(hpcgx_ctx *ctx, int level, uint8_t *nnz_row, int *cols, double *vals);
+/* Builds the fp32 or scaled-fp16 copies of every level for the mixed solvers. */
+int hpcgx_prepare_precision(hpcgx_ctx *ctx, int precision);
+/* Device bytes held by the context (OptimizeProblemMemoryUse, OptimizeProblem.cpp:103-107). */
+double hpcgx_memory_bytes(const hpcgx_ctx *ctx);
+
+/* ---- vectors (Vector.hpp:27-113) --------------------------------------- */
+
+/* Device vector of one level (padded length), zero-filled. */
+int hpcgx_vec_alloc(hpcgx_ctx *ctx, int level, double **dvec);
+int hpcgx_vec_free(hpcgx_ctx *ctx, double *dvec);
+/* host (natural order, n entries) -> device (colour order) */
+int hpcgx_vec_upload(hpcgx_ctx *ctx, int level, double *dvec, const double *host, int64_t n);
+/* device (colour order) -> host (natural order, n entries) */
+int hpcgx_vec_download(hpcgx_ctx *ctx, int level, const double *dvec, double *host, int64_t n);
+/* ZeroVector (Vector.hpp:55-60) and friends: every true entry := value. */
+int hpcgx_vec_fill(hpcgx_ctx *ctx, int level, double *dvec, double value);
+
+/* ---- kernels ------------------------------------------------------------ */
+
+/* y = A x on one level (ComputeSPMV, ComputeSPMV.cpp:40-45; arithmetic of
+ * ComputeSPMV_ref.cpp:61-70, summed in the same order: bit-exact). */
+int hpcgx_spmv(hpcgx_ctx *ctx, int level, const double *x, double *y);
+/* One symmetric Gauss-Seidel step, 8 colours forward then backward
+ * (ComputeSYMGS, ComputeSYMGS.cpp:50-55; row arithmetic of
+ * ComputeSYMGS_ref.cpp:67-100). */
+int hpcgx_symgs(hpcgx_ctx *ctx, int level, const double *r, double *x);
+/* rc = (r - A x) at the injected rows of `level`, written in level+1 order
+ * (ComputeSPMV_ref + ComputeRestriction_ref.cpp:51, fused: only the
+ * injected rows' residual is formed). */
+int hpcgx_restrict(hpcgx_ctx *ctx, int level, const double *r, const double *x, double *rc);
+/* x[f2c] += xc (ComputeProlongation_ref.cpp:49); xc in level+1 order. */
+int hpcgx_prolong(hpcgx_ctx *ctx, int level, const double *xc, double *x);
+/* V-cycle from `level` down (ComputeMG, ComputeMG.cpp:33-38; structure of
+ * ComputeMG_ref.cpp:39-63). */
+int hpcgx_mg(hpcgx_ctx *ctx, int level, const double *r, double *x);
+/* The V-cycle in reduced precision (no reference counterpart): x = MG(r) with
+ * r rounded to fp32, the fp32 (HPCGX_PREC_FP32) or diagonally scaled fp16
+ * (HPCGX_PREC_FP16) copies of every level, fp32 arithmetic, x widened to fp64.
+ * Level-0 device vectors; needs hpcgx_prepare_precision. */
+int hpcgx_mg_mixed(hpcgx_ctx *ctx, int precision, const double *r, double *x);
+/* result = x . y over n entries of a level-0 vector (ComputeDotProduct,
+ * ComputeDotProduct.cpp:41-47).  Fixed-order reduction: the same inputs give
+ * the same bits on every run. */
+int hpcgx_dot(hpcgx_ctx *ctx, int64_t n, const double *x, const double *y, double *result);
+/* w = alpha x + beta y (ComputeWAXPBY, ComputeWAXPBY.cpp:42-48; bit-exact). */
+int hpcgx_waxpby(hpcgx_ctx *ctx, int64_t n, double alpha, const double *x, double beta,
+                 const double *y, double *w);
+/* w = alpha x + beta y and result = w . w in one pass (no reference counterpart;
+ * the pair WAXPBY+DOT of CG.cpp:122-123). */
+int hpcgx_waxpby_dot(hpcgx_ctx *ctx, int64_t n, double alpha, const double *x, double beta,
+                     const double *y, double *w, double *result);
+
+/* ---- solver (CG, CG.cpp:59-143) ---------------------------------------- */
+
+/* Preconditioned CG on level 0.  b, x: device vectors (colour order); x holds
+ * the initial guess and receives the solution.  HPCGX_CG_FP64 follows the
+ * CG.cpp recurrence exactly (stops when normr/normr0 <= tol or after
+ * max_iter).  The mixed variants run a flexible (Polak-Ribiere) PCG with an
+ * fp32 / scaled-fp16 MG and fp64 outer vectors, stop when the fp64 true
+ * residual ||b - A x|| / normr0 <= tol, and need hpcgx_prepare_precision.
+ * times: NULL or the reference's times[] (CG.cpp:133-141 slots 0..5,
+ * accumulated).  hist: NULL or max_iter+1 doubles, normr after each
+ * iteration (hist[0] = normr0). */
+int hpcgx_cg(hpcgx_ctx *ctx, int variant, const double *b, double *x, int max_iter, double tol,
+             int do_preconditioning, int *niters, double *normr, double *normr0, double *times,
+             double *hist);
+
+/* ---- variable-coefficient adjoint (no reference counterpart) ----------- */
+
+/* g = dL/dkappa for L(x(kappa)) given the adjoint lam = A^-1 dL/dx:
+ * g_k = -1/2 sum_{j in N(k)} (lam_k - lam_j)(x_k - x_j) - (26 - n_k) lam_k x_k.
+ * lam, x: level-0 device vectors; g: device vector (colour order). */
+int hpcgx_kappa_grad(hpcgx_ctx *ctx, const double *lam, const double *x, double *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,53 @@
+"""CPU-side checks of the boundary: the HIP library loads and exports every
+entry point declared in include/hpcgx.h; the C++ drop-in library exports the
+reference's hook symbols; the FLOP / byte models reproduce the reference's
+closed forms.  No device calls."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"\b(hpcgx_[a-z0-9_]+)\s*\(", text)))
+
+
+def exported(so):
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True, text=True, check=True)
+    return {l.split()[-1] for l in out.stdout.splitlines() if l.strip()}
+
+
+def test_library_loads_and_exports_header(hip):
+    from hpcg_amd import _lib
+    syms = exported(_lib.LIB_PATH)
+    decl = declared("hpcgx.h")
+    assert len(decl) >= 25
+    missing = [s for s in decl if s not in syms]
+    assert not missing, missing
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for s in decl:
+        assert hasattr(L, s)
+    assert set(_lib.SIGNATURES) <= set(decl)
+
+
+def test_null_context_is_an_error(hip):
+    L = hip.lib()
+    assert L.hpcgx_error_string(None) == b"null context"
+    assert L.hpcgx_destroy(None) == -1
+    assert L.hpcgx_sync(None) == -1
+
+
+def test_flop_model_matches_reference_closed_form():
+    from hpcg_amd.models import hpcg_flops, level_shapes
+    sh = level_shapes(192, 192, 192)
+    assert [s[1] for s in sh] == [189119224, 23393656, 2863288, 343000]
+    n, m = sh[0][0], sh[0][1]
+    per_it = 12 * n + 2 * m + 10 * sum(s[1] for s in sh[:3]) + 4 * sh[3][1]
+    f = hpcg_flops(sh, iters=50, sets=1)
+    assert f["total"] == pytest.approx(50 * per_it + 4 * n + 2 * m, rel=1e-15)
+    assert per_it / 1e9 == pytest.approx(2.618, abs=5e-4)

@@ -1,0 +1,196 @@
+"""Solver-level parity through the C ABI: the CG recurrence against the oracle,
+the reference's validation tests (TestCG, TestSymmetry, TestNorms) restated
+over the device kernels, the mixed-precision V-cycles and solvers, and the
+variable-coefficient operator with its gradient."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+def device_problem(hip, n, levels=4, kappa=None):
+    ctx = hip.Context(0)
+    ctx.generate(n, n, n, levels=levels, kappa=kappa)
+    return ctx
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_cg_history_matches_oracle(hip, n):
+    """Same recurrence (CG.cpp:86-130) and kernels; the dots' summation order
+    is the only difference, so the residual histories agree to ~1e-12 while
+    they are well above rounding level."""
+    ctx = device_problem(hip, n)
+    P = po.Problem(n, n, n)
+    b, _, _ = P.rhs()
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    got = ctx.cg(db, dx, max_iter=50, tol=0.0, history=True)
+    want = P.cg(b, max_iter=50, mode=2, history=True)
+    assert got["niters"] == want["niters"] == 50
+    assert got["normr0"] == want["normr0"] or abs(got["normr0"] / want["normr0"] - 1) < 1e-14
+    h, hw = got["hist"] / got["normr0"], want["hist"] / want["normr0"]
+    live = hw > 1e-10
+    assert np.allclose(h[live], hw[live], rtol=1e-8, atol=0)
+    x = dx.download()
+    assert np.max(np.abs(x - 1.0)) < 1e-8   # converged to the exact solution (all ones)
+
+
+@pytest.mark.parametrize("n,ref_reduction", [(16, 3.66234e-41), (32, 1.0298266781532878e-20)])
+def test_cg_reaches_reference_reduction(hip, n, ref_reduction):
+    """The reference (natural-order Gauss-Seidel) reaches `ref_reduction`
+    after 50 iterations; the coloured path must reach it within 50 +/- 2."""
+    ctx = device_problem(hip, n)
+    b = po.Problem(n, n, n).rhs()[0]
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    out = ctx.cg(db, dx, max_iter=52, tol=ref_reduction * 1.0000001)
+    assert out["normr"] / out["normr0"] <= ref_reduction * 1.0000001
+    assert out["niters"] <= 52
+
+
+def _testcg(ctx, P, b):
+    """TestCG.cpp:62-115 over the device: exaggerate the diagonal and b, then
+    2 unpreconditioned and 2 preconditioned solves at tol 1e-12."""
+    n = P.n
+    orig = ctx.get_diagonal(0)
+    assert np.array_equal(orig, P.diagonal())
+    scale = np.full(n, 1.0e6)
+    gid = np.arange(n)
+    scale[gid < 9] = (gid[gid < 9] + 2) * 1.0e6
+    ctx.set_diagonal(0, orig * scale)
+    db = ctx.vec(0, b * scale)
+    dx = ctx.vec(0)
+    iters = {}
+    for prec in (False, True):
+        iters[prec] = []
+        for _ in range(2):
+            ctx.fill(dx, 0.0)
+            out = ctx.cg(db, dx, max_iter=50, tol=1e-12, precond=prec)
+            iters[prec].append(out["niters"])
+    ctx.set_diagonal(0, orig)
+    assert np.array_equal(ctx.get_diagonal(0), orig)
+    return iters
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_testcg(hip, n):
+    ctx = device_problem(hip, n)
+    P = po.Problem(n, n, n)
+    it = _testcg(ctx, P, P.rhs()[0])
+    assert max(it[False]) <= 12 and max(it[True]) <= 2, it
+
+
+def _departure(a, b, xn2, yn2):
+    eps = np.finfo(float).eps
+    anorm = 2 * 26.0
+    return abs(a - b) / ((xn2 * anorm * yn2 + yn2 * anorm * xn2) * eps)
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_testsymmetry(hip, n):
+    """TestSymmetry.cpp:76-132: x'Ay vs y'Ax and x'M^-1y vs y'M^-1x, scaled
+    departure <= 1; A * xexact == b exactly."""
+    ctx = device_problem(hip, n)
+    P = po.Problem(n, n, n)
+    x = rng(1).random(P.n) + 1.0
+    y = rng(2).random(P.n) + 1.0
+    dx, dy, dz = ctx.vec(0, x), ctx.vec(0, y), ctx.vec(0)
+    xn2, yn2 = ctx.dot(dx, dx), ctx.dot(dy, dy)
+    ctx.spmv(0, dy, dz)
+    xAy = ctx.dot(dx, dz)
+    ctx.spmv(0, dx, dz)
+    yAx = ctx.dot(dy, dz)
+    assert _departure(xAy, yAx, xn2, yn2) <= 1.0
+    ctx.mg(0, dy, dz)
+    xMy = ctx.dot(dx, dz)
+    ctx.mg(0, dx, dz)
+    yMx = ctx.dot(dy, dz)
+    assert _departure(xMy, yMx, xn2, yn2) <= 1.0
+    b, _, xe = P.rhs()
+    dx.upload(xe)
+    ctx.spmv(0, dx, dz)
+    assert np.max(np.abs(dz.download() - b)) == 0.0
+
+
+def test_testnorms(hip):
+    """TestNorms.cpp:31-44: repeated CG sets give identical scaled residuals."""
+    ctx = device_problem(hip, 32)
+    b = po.Problem(32, 32, 32, levels=1).rhs()[0]
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    vals = []
+    for _ in range(3):
+        ctx.fill(dx, 0.0)
+        out = ctx.cg(db, dx, max_iter=50, tol=0.0)
+        vals.append(out["normr"] / out["normr0"])
+    assert np.var(vals) == 0.0
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+@pytest.mark.parametrize("n", [16, 32])
+def test_mixed_mg_bitexact(hip, prec, n):
+    ctx = device_problem(hip, n)
+    P = po.Problem(n, n, n)
+    code = hip.HPCGX_PREC_FP32 if prec == "fp32" else hip.HPCGX_PREC_FP16
+    ctx.prepare(code)
+    r = rng(3).random(P.n) - 0.5
+    dr, dz = ctx.vec(0, r), ctx.vec(0)
+    ctx.mg_mixed(code, dr, dz)
+    r32 = r.astype(np.float32)
+    want = (P.mg_f32(r32) if prec == "fp32" else P.mg_f16(r32)).astype(np.float64)
+    assert np.array_equal(dz.download(), want)
+
+
+@pytest.mark.parametrize("variant", ["fp32", "fp16"])
+def test_mixed_cg_converges(hip, variant):
+    n = 32
+    ctx = device_problem(hip, n)
+    P = po.Problem(n, n, n)
+    b = P.rhs()[0]
+    code = hip.HPCGX_PREC_FP32 if variant == "fp32" else hip.HPCGX_PREC_FP16
+    cgv = hip.HPCGX_CG_MIXED_FP32 if variant == "fp32" else hip.HPCGX_CG_MIXED_FP16
+    ctx.prepare(code)
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    out = ctx.cg(db, dx, max_iter=200, tol=1e-9, variant=cgv)
+    x = dx.download()
+    true_rel = np.linalg.norm(b - P.spmv(x)) / np.linalg.norm(b)
+    assert true_rel <= 1e-9
+    want = P.cg_mixed(b, max_iter=200, tol=1e-9, mg_prec=32 if variant == "fp32" else 16)
+    assert abs(out["niters"] - want["niters"]) <= 1
+    fp64 = P.cg(b, max_iter=200, tol=1e-9, mode=2)
+    assert out["niters"] <= fp64["niters"] + 3
+
+
+def _kappa(n, seed=7):
+    g = rng(seed)
+    return np.exp(0.5 * g.standard_normal(n ** 3))
+
+
+def test_kappa_operator_and_gradient_bitexact(hip):
+    n = 16
+    kappa = _kappa(n)
+    ctx = device_problem(hip, n, kappa=kappa)
+    P = po.Problem(n, n, n, kappa=kappa)
+    for l in range(4):
+        onr, ocols, ovals, _ = P.level(l).arrays()
+        nr, cols, vals = ctx.export_level(l)
+        mask = np.arange(27)[None, :] < onr[:, None]
+        assert np.array_equal(nr, onr)
+        assert np.array_equal(cols[mask], ocols[mask])
+        assert np.array_equal(vals[mask], ovals[mask])
+    lam = rng(8).standard_normal(P.n)
+    x = rng(9).standard_normal(P.n)
+    dl, dx, dg = ctx.vec(0, lam), ctx.vec(0, x), ctx.vec(0)
+    ctx.kappa_grad(dl, dx, dg)
+    assert np.array_equal(dg.download(), P.kappa_grad(lam, x))
+
+
+def test_kappa_one_is_hpcg_matrix(hip):
+    n = 16
+    a = device_problem(hip, n).export_level(0)
+    b = device_problem(hip, n, kappa=np.ones(n ** 3)).export_level(0)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)

@@ -1,0 +1,178 @@
+"""Kernel-level parity of the HIP path (through the C ABI) against the oracle.
+
+Bars: bit-exact for the matrix, SpMV, coloured SYMGS, restriction,
+prolongation, the V-cycle and WAXPBY (same per-row summation order, no FMA on
+either side); dot products within 1e-13 relative (only the summation order
+differs).  The oracle's natural-order functions are themselves bit-exact
+against the reference build (tests/test_oracle.py).
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(16, 16, 16), (24, 24, 24), (32, 16, 24)]
+
+
+@pytest.fixture(scope="module")
+def problems(hip):
+    out = {}
+    for g in GRIDS:
+        ctx = hip.Context(0)
+        ctx.generate(*g, levels=4)
+        out[g] = (ctx, po.Problem(*g, levels=4))
+    yield out
+    for ctx, _ in out.values():
+        ctx.close()
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+@pytest.mark.parametrize("g", GRIDS)
+def test_generated_matrix_equals_reference_layout(problems, g):
+    ctx, P = problems[g]
+    for l in range(4):
+        lv = P.level(l)
+        onr, ocols, ovals, _ = lv.arrays()
+        nr, cols, vals = ctx.export_level(l)
+        assert ctx.info(l)["nnz"] == lv.nnz
+        assert np.array_equal(nr, onr)
+        mask = np.arange(27)[None, :] < onr[:, None]
+        assert np.array_equal(cols[mask], ocols[mask])
+        assert np.array_equal(vals[mask], ovals[mask])
+
+
+def test_upload_equals_generate(hip):
+    P = po.Problem(16, 16, 16)
+    up = hip.Context(0)
+    for l in range(4):
+        lv = P.level(l)
+        nr, cols, vals, f2c = lv.arrays()
+        up.upload_level(l, lv.dims, nr, cols, vals, f2c)
+    gen = hip.Context(0).generate(16, 16, 16)
+    for l in range(4):
+        a, b = up.export_level(l), gen.export_level(l)
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+    x = rng(5).random(P.n) + 1.0
+    ys = []
+    for ctx in (up, gen):
+        dx, dy = ctx.vec(0, x), ctx.vec(0)
+        ctx.spmv(0, dx, dy)
+        ys.append(dy.download())
+    assert np.array_equal(ys[0], ys[1])
+
+
+def test_upload_rejects_non_stencil(hip):
+    P = po.Problem(8, 8, 8, levels=1)
+    nr, cols, vals, _ = P.arrays()
+    cols = cols.copy()
+    cols[100, 0] = (cols[100, 0] + 50) % P.n   # a far-away column
+    ctx = hip.Context(0)
+    with pytest.raises(hip.HpcgxError, match="ENOTSTENCIL"):
+        ctx.upload_level(0, P.dims, nr, cols, vals, None)
+
+
+@pytest.mark.parametrize("g", GRIDS)
+def test_spmv_bitexact(problems, g):
+    ctx, P = problems[g]
+    for l in range(4):
+        lv = P.level(l)
+        x = rng(l + 1).random(lv.n) + 1.0
+        dx, dy = ctx.vec(l, x), ctx.vec(l)
+        ctx.spmv(l, dx, dy)
+        assert np.array_equal(dy.download(), lv.spmv(x)), f"level {l}"
+        dx.free(), dy.free()
+
+
+def test_spmv_times_ones_is_b(problems):
+    ctx, P = problems[(24, 24, 24)]
+    b, _, xe = P.rhs()
+    dx, dy = ctx.vec(0, xe), ctx.vec(0)
+    ctx.spmv(0, dx, dy)
+    assert np.array_equal(dy.download(), b)
+
+
+@pytest.mark.parametrize("g", GRIDS)
+def test_symgs_bitexact(problems, g):
+    ctx, P = problems[g]
+    for l in range(4):
+        lv = P.level(l)
+        r = rng(10 + l).random(lv.n)
+        x0 = rng(20 + l).random(lv.n)
+        dr, dx = ctx.vec(l, r), ctx.vec(l, x0)
+        ctx.symgs(l, dr, dx)
+        got = dx.download()
+        assert np.array_equal(got, lv.symgs(r, x0, mode=2)), f"level {l}"
+        # against the full coloured sweep (backward step of the last colour
+        # performed): a few ulp apart
+        full = lv.symgs(r, x0, mode=1)
+        assert np.max(np.abs(got - full) / np.abs(full)) < 1e-14
+        dr.free(), dx.free()
+
+
+@pytest.mark.parametrize("g", GRIDS)
+def test_restrict_prolong_bitexact(problems, g):
+    ctx, P = problems[g]
+    for l in range(3):
+        lv, cv = P.level(l), P.level(l + 1)
+        r = rng(30 + l).random(lv.n)
+        x = rng(40 + l).random(lv.n)
+        xc = rng(50 + l).random(cv.n)
+        dr, dx, drc, dxc = ctx.vec(l, r), ctx.vec(l, x), ctx.vec(l + 1), ctx.vec(l + 1, xc)
+        ctx.restrict(l, dr, dx, drc)
+        assert np.array_equal(drc.download(), lv.restrict(r, lv.spmv(x)))
+        ctx.prolong(l, dxc, dx)
+        assert np.array_equal(dx.download(), lv.prolong(xc, x))
+        for v in (dr, dx, drc, dxc):
+            v.free()
+
+
+@pytest.mark.parametrize("g", GRIDS)
+def test_mg_bitexact(problems, g):
+    ctx, P = problems[g]
+    r = rng(60).random(P.n)
+    dr, dz = ctx.vec(0, r), ctx.vec(0)
+    want = P.mg(r, mode=2)
+    for _ in range(2):  # the V-cycle must not depend on what the buffers held before
+        ctx.mg(0, dr, dz)
+        assert np.array_equal(dz.download(), want)
+    dr.free(), dz.free()
+
+
+def test_dot_and_waxpby(problems):
+    ctx, P = problems[(32, 16, 24)]
+    x = rng(70).random(P.n) + 1.0
+    y = rng(71).random(P.n) - 0.5
+    dx, dy, dw = ctx.vec(0, x), ctx.vec(0, y), ctx.vec(0)
+    d = ctx.dot(dx, dy)
+    ref = po.dot(x, y)
+    assert abs(d - ref) <= 1e-13 * abs(ref)
+    assert abs(ctx.dot(dx, dx) - po.dot(x, x)) <= 1e-13 * po.dot(x, x)
+    assert ctx.dot(dx, dy) == d  # fixed-order reduction: identical bits every run
+    for alpha, beta in ((1.0, -1.0), (0.75, 1.0), (-0.3, 2.5), (1.0, 0.0)):
+        ctx.waxpby(alpha, dx, beta, dy, dw)
+        assert np.array_equal(dw.download(), po.waxpby(alpha, x, beta, y))
+        nrm = ctx.waxpby_dot(alpha, dx, beta, dy, dw)
+        w = po.waxpby(alpha, x, beta, y)
+        assert np.array_equal(dw.download(), w)
+        assert abs(nrm - po.dot(w, w)) <= 1e-13 * po.dot(w, w)
+    if po.ref_available():
+        assert abs(d - po.ref_dot(x, y)) <= 1e-13 * abs(ref)
+        assert np.array_equal(po.ref_waxpby(0.75, x, 1.0, y), po.waxpby(0.75, x, 1.0, y))
+
+
+def test_vector_roundtrip_and_fill(problems):
+    ctx, P = problems[(32, 16, 24)]
+    x = rng(80).random(P.n)
+    dx = ctx.vec(0, x)
+    assert np.array_equal(dx.download(), x)
+    ctx.fill(dx, 0.0)
+    assert not dx.download().any()
+    # padding stays zero: dot over the padded vector equals the true dot
+    dx.upload(np.ones(P.n))
+    assert ctx.dot(dx, dx) == float(P.n)
