@@ -29,6 +29,7 @@ SIGNATURES = {
     "hpcgx_error_string": (C.c_char_p, [_vp]),
     "hpcgx_set_stream": (C.c_int, [_vp, _vp]),
     "hpcgx_sync": (C.c_int, [_vp]),
+    "hpcgx_set_colour_order": (C.c_int, [_vp, _ip]),
     "hpcgx_level_upload": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "hpcgx_level_upload_packed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _u8p, _ip, _dp, _vp]),
     "hpcgx_problem_generate": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
@@ -145,6 +146,9 @@ class Context:
         return rc
 
     # ---- problem
+    def set_colour_order(self, order):
+        self._call("hpcgx_set_colour_order", self.h, np.ascontiguousarray(order, np.int32))
+
     def generate(self, nx, ny, nz, levels=4, kappa=None):
         kp = None
         if kappa is not None:

@@ -18,23 +18,19 @@ static inline unsigned tiles(int64_t rows) { return (unsigned)((rows + kBlock - 
 
 template <class ZT>
 __global__ __launch_bounds__(kBlock) void k_cg_rz(int64_t n, const double *__restrict__ r,
-                                                  const ZT *__restrict__ z, double *partials,
-                                                  unsigned *ticket, double *scal, int first) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-    acc += r[i] * (double)z[i];
-  double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, ticket, tot)) {
-    if (first) {
-      scal[S_RTZ] = tot[0];
-    } else {
-      const double old = scal[S_RTZ];
-      scal[S_OLDRTZ] = old;
-      scal[S_RTZ] = tot[0];
-      scal[S_BETA] = tot[0] / old;
-    }
+                                                  const ZT *__restrict__ z, double *partials) {
+  double a[kChunk], b[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    a[j] = i < n ? r[i] : 0.0;
+    b[j] = i < n ? (double)z[i] : 0.0;
   }
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) acc += a[j] * b[j];
+  const double v[1] = {acc};
+  block_partials<1>(v, partials);
 }
 
 // p = z + beta p  (first: z + 0.0 z, the WAXPBY(1,z,0,z) of CG.cpp:109)
@@ -47,104 +43,113 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict
   p[i] = first ? zi + 0.0 * zi : zi + scal[S_BETA] * p[i];
 }
 
-// Ap = A p and pAp = p.Ap ; alpha = rtz / pAp.  Grid-stride over 256-row
-// tiles with a fixed grid, so the reduction order is fixed.
+// Ap = A p and the partials of pAp = p.Ap.  One 256-row tile per block.
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const int *__restrict__ cols,
                                                         const double *__restrict__ vals,
                                                         const double *__restrict__ p,
                                                         double *__restrict__ Ap, int64_t nrows,
-                                                        double *partials, unsigned *ticket, double *scal) {
+                                                        double *partials) {
+  const int64_t row = tile_id() * kBlock + threadIdx.x;
   double acc = 0.0;
-  const int64_t ntiles = (nrows + kBlock - 1) / kBlock;
-  for (int64_t tile = tile_id(); tile < ntiles; tile += gridDim.x) {
-    const int64_t row = tile * kBlock + threadIdx.x;
-    if (row < nrows) {
-      const size_t b = ell_base(row);
-      double sum = 0.0;
+  if (row < nrows) {
+    const size_t b = ell_base(row);
+    double sum = 0.0;
 #pragma unroll
-      for (int s = 0; s < kSlots; ++s) {
-        const int c = ld_col(cols + b + s * kSlice);
-        const double v = ld_val<double>(vals + b + s * kSlice);
-        sum += v * p[c];
-      }
-      Ap[row] = sum;
-      acc += p[row] * sum;
+    for (int s = 0; s < kSlots; ++s) {
+      const int c = ld_col(cols + b + s * kSlice);
+      const double v = ld_val<double>(vals + b + s * kSlice);
+      sum += v * p[c];
     }
+    Ap[row] = sum;
+    acc = p[row] * sum;
   }
-  double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, ticket, tot)) {
-    scal[S_PAP] = tot[0];
-    scal[S_ALPHA] = scal[S_RTZ] / tot[0];
-  }
+  const double v[1] = {acc};
+  block_partials<1>(v, partials);
 }
 
-// x += alpha p ; r -= alpha Ap ; normr2 = r.r ; hist[k] = normr2
+// x += alpha p ; r -= alpha Ap ; normr2 = r.r
 // R32: also leave r in fp32 for the next mixed-precision V-cycle.
 template <bool R32>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restrict__ x,
                                                       const double *__restrict__ p, double *__restrict__ r,
                                                       const double *__restrict__ Ap, float *__restrict__ r32,
-                                                      double *partials, unsigned *ticket, double *scal,
-                                                      double *hist, int k) {
+                                                      double *partials, const double *__restrict__ scal) {
   const double alpha = scal[S_ALPHA], malpha = -alpha;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double xv[kChunk], pv[kChunk], rv[kChunk], qv[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    const bool in = i < n;
+    xv[j] = in ? x[i] : 0.0;
+    pv[j] = in ? p[i] : 0.0;
+    rv[j] = in ? r[i] : 0.0;
+    qv[j] = in ? Ap[i] : 0.0;
+  }
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    x[i] = x[i] + alpha * p[i];
-    const double ri = r[i] + malpha * Ap[i];
-    r[i] = ri;
-    if (R32) r32[i] = (float)ri;
-    acc += ri * ri;
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    const double ri = rv[j] + malpha * qv[j];
+    if (i < n) {
+      x[i] = xv[j] + alpha * pv[j];
+      r[i] = ri;
+      if (R32) r32[i] = (float)ri;
+      acc += ri * ri;
+    }
   }
-  double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, ticket, tot)) {
-    scal[S_NORMR2] = tot[0];
-    if (hist) hist[k] = tot[0];
-  }
+  const double v[1] = {acc};
+  block_partials<1>(v, partials);
 }
 
 // r = b - Ap ; normr2 = r.r (CG.cpp:88-89) ; optional fp32 copy of r
 template <bool R32>
 __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double *__restrict__ b,
                                                         const double *__restrict__ Ap, double *__restrict__ r,
-                                                        float *__restrict__ r32, double *partials,
-                                                        unsigned *ticket, double *scal) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const double ri = b[i] + -1.0 * Ap[i];
-    r[i] = ri;
-    if (R32) r32[i] = (float)ri;
-    acc += ri * ri;
+                                                        float *__restrict__ r32, double *partials) {
+  double bv[kChunk], qv[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    bv[j] = i < n ? b[i] : 0.0;
+    qv[j] = i < n ? Ap[i] : 0.0;
   }
-  double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, ticket, tot)) scal[S_NORMR2] = tot[0];
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    const double ri = bv[j] + -1.0 * qv[j];
+    if (i < n) {
+      r[i] = ri;
+      if (R32) r32[i] = (float)ri;
+      acc += ri * ri;
+    }
+  }
+  const double v[1] = {acc};
+  block_partials<1>(v, partials);
 }
 
 // Flexible CG (Polak-Ribiere): rtz = r.z ; zAp = z.Ap_{k-1} ;
 // beta = -alpha_{k-1} zAp / oldrtz, since r_k - r_{k-1} = -alpha_{k-1} A p_{k-1}.
 __global__ __launch_bounds__(kBlock) void k_fcg_rz(int64_t n, const double *__restrict__ r,
                                                    const float *__restrict__ z, const double *__restrict__ Ap,
-                                                   double *partials, unsigned *ticket, double *scal,
-                                                   int restart) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
+                                                   double *partials, int restart) {
+  double rv[kChunk], zv[kChunk], qv[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    const bool in = i < n;
+    rv[j] = in ? r[i] : 0.0;
+    zv[j] = in ? (double)z[i] : 0.0;
+    qv[j] = (in && !restart) ? Ap[i] : 0.0;
+  }
   double a0 = 0.0, a1 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const double zi = (double)z[i];
-    a0 += r[i] * zi;
-    if (!restart) a1 += zi * Ap[i];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    a0 += rv[j] * zv[j];
+    a1 += zv[j] * qv[j];
   }
-  double v[2] = {a0, a1}, tot[2];
-  if (grid_reduce<2>(v, partials, ticket, tot)) {
-    if (restart) {
-      scal[S_RTZ] = tot[0];
-    } else {
-      const double old = scal[S_RTZ];
-      scal[S_OLDRTZ] = old;
-      scal[S_RTZ] = tot[0];
-      scal[S_BETA] = -scal[S_ALPHA] * tot[1] / old;
-    }
-  }
+  const double v[2] = {a0, a1};
+  block_partials<2>(v, partials);
 }
 
 // x[...] copy for p = z on restart (fp32 z)
@@ -226,14 +231,15 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
   const double t_begin = now_s();
   Level &L0 = c->levels[0];
   const int64_t np = L0.lay.npad;
-  const int G = reduce_grid_size(np);
+  const unsigned G = chunk_blocks(np), T = tiles(np);
   double *r = c->cg_r, *z = precond ? c->cg_z : c->cg_r, *p = c->cg_p, *Ap = c->cg_Ap;
   int rc;
   PhaseTimer tm{c, times != nullptr};
   // preamble: Ap = A x ; r = b - Ap ; normr = ||r||  (CG.cpp:86-90)
   if ((rc = spmv_fp64(c, 0, x, Ap))) return rc;
-  k_cg_residual<false><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, nullptr, c->partials, c->ticket, c->scal);
+  k_cg_residual<false><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, nullptr, c->partials);
   HPCGX_CHECK_LAUNCH(c);
+  if ((rc = finish(c, 1, G, F_NORMR2))) return rc;
   double n2;
   if ((rc = read_scalar(c, S_NORMR2, &n2))) return rc;
   double normr = std::sqrt(n2);
@@ -244,14 +250,16 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
     tm.mark(0);
     if (precond && (rc = mg_fp64(c, 0, r, z))) return rc;
     tm.mark(1);
-    k_cg_rz<double><<<G, kBlock, 0, c->stream>>>(np, r, z, c->partials, c->ticket, c->scal, k == 1);
+    k_cg_rz<double><<<G, kBlock, 0, c->stream>>>(np, r, z, c->partials);
+    finish(c, 1, G, k == 1 ? F_RZ_FIRST : F_RZ);
     tm.mark(2);
     k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, k == 1);
     tm.mark(3);
-    k_cg_spmv_dot<<<G, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials, c->ticket, c->scal);
+    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials);
+    finish(c, 1, T, F_PAP);
     tm.mark(4);
-    k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->ticket, c->scal,
-                                                    nullptr, k);
+    k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->scal);
+    finish(c, 1, G, F_NORMR2);
     tm.mark(5);
     HPCGX_CHECK_LAUNCH(c);
     if ((rc = read_scalar(c, S_NORMR2, &n2))) return rc;
@@ -274,7 +282,7 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
   const double t_begin = now_s();
   Level &L0 = c->levels[0];
   const int64_t np = L0.lay.npad;
-  const int G = reduce_grid_size(np);
+  const unsigned G = chunk_blocks(np), T = tiles(np);
   double *r = c->cg_r, *p = c->cg_p, *Ap = c->cg_Ap;
   float *r32 = c->cg_r32, *z32 = c->cg_z32;
   int rc;
@@ -282,8 +290,9 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
   auto true_residual = [&](double *n2) -> int {
     int e = spmv_fp64(c, 0, x, Ap);
     if (e) return e;
-    k_cg_residual<true><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, r32, c->partials, c->ticket, c->scal);
+    k_cg_residual<true><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, r32, c->partials);
     HPCGX_CHECK_LAUNCH(c);
+    if ((e = finish(c, 1, G, F_NORMR2))) return e;
     return read_scalar(c, S_NORMR2, n2);
   };
   double n2;
@@ -309,7 +318,8 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
       rc = mg_fp32(c, 0, r32, z32);
     if (rc) return rc;
     tm.mark(1);
-    k_fcg_rz<<<G, kBlock, 0, c->stream>>>(np, r, z32, Ap, c->partials, c->ticket, c->scal, restart ? 1 : 0);
+    k_fcg_rz<<<G, kBlock, 0, c->stream>>>(np, r, z32, Ap, c->partials, restart ? 1 : 0);
+    finish(c, 2, G, restart ? F_RZ_FIRST : F_FCG);
     tm.mark(2);
     if (restart)
       k_copy_z<<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p);
@@ -317,10 +327,11 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
       k_cg_p<float><<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, c->scal, 0);
     restart = false;
     tm.mark(3);
-    k_cg_spmv_dot<<<G, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials, c->ticket, c->scal);
+    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials);
+    finish(c, 1, T, F_PAP);
     tm.mark(4);
-    k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->ticket, c->scal,
-                                                   nullptr, k);
+    k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal);
+    finish(c, 1, G, F_NORMR2);
     tm.mark(5);
     HPCGX_CHECK_LAUNCH(c);
     if ((rc = read_scalar(c, S_NORMR2, &n2))) return rc;

@@ -18,7 +18,6 @@ constexpr int kSlice = 64;     // rows per ELL slice = one wavefront
 constexpr int kDiagSlot = 13;  // slot of offset (0,0,0)
 constexpr int kBlock = 256;    // threads per block = 4 slices
 constexpr int kColours = 8;
-constexpr int kReduceBlocks = 2048;  // fixed grid of reducing kernels (determinism)
 constexpr int kMaxLevels = 8;
 
 // Device scalars of the CG recurrences (Ctx::scal).
@@ -34,10 +33,20 @@ enum Scalar : int {
   S_COUNT = 16
 };
 
+// Sweep order: the parity class ((x&1)|(y&1)<<1|(z&1)<<2) of block k.  The
+// injected rows (all coordinates even, class 0) are updated last in the
+// forward sweep and first in the backward sweep: swept last in the backward
+// sweep, their residual would be zero and the V-cycle would lose its
+// coarse-grid correction.  Between the all-odd class 7 and class 0 come the
+// complementary pairs (3,4), (5,2), (6,1).  Of the 840 orders with class 0
+// last (up to axis permutations) this one converges fastest on the 192^3
+// problem: it reaches the reference's 50-iteration residual reduction in 50
+// iterations, and relative residual 1e-9 in fewer iterations than the
+// reference's natural-order Gauss-Seidel at 32^3..96^3.
+constexpr int kDefaultOrder[kColours] = {7, 3, 4, 5, 2, 6, 1, 0};
+
 // Row layout of one level.  Rows are grouped in 8 blocks, block k holding the
-// rows of parity class 7-k ((x&1)|(y&1)<<1|(z&1)<<2), so the forward sweep
-// visits classes 7,6,...,0 and the injected rows (all coordinates even,
-// class 0) are the last block.  Inside a block rows keep natural order of
+// rows of parity class cls[k].  Inside a block rows keep natural order of
 // (x>>1, y>>1, z>>1).  Each block is padded to a multiple of 64 rows; padded
 // rows hold the identity row (diagonal 1, other slots 0) and zero vectors.
 struct Layout {
@@ -46,9 +55,12 @@ struct Layout {
   int64_t start[kColours + 1] = {};  // padded block starts
   int64_t count[kColours] = {};      // true rows per block
   int nb[kColours][3] = {};          // block extents in x, y, z
+  int cls[kColours] = {};            // parity class of block k
+  int pos[kColours] = {};            // block of parity class c
+  int inj = 0;                       // block of class 0 (the injected rows)
 };
 
-Layout make_layout(int nx, int ny, int nz);
+Layout make_layout(int nx, int ny, int nz, const int *order);
 
 __host__ __device__ inline int parity_class(int ix, int iy, int iz) {
   return (ix & 1) | ((iy & 1) << 1) | ((iz & 1) << 2);
@@ -58,11 +70,12 @@ struct LayoutDev {  // by-value kernel argument
   int nx, ny, nz;
   int64_t start[kColours + 1];
   int nb[kColours][3];
+  int cls[kColours];
+  int pos[kColours];
 };
 
 __host__ __device__ inline int64_t perm_of(const LayoutDev &L, int ix, int iy, int iz) {
-  const int c = parity_class(ix, iy, iz);
-  const int k = 7 - c;
+  const int k = L.pos[parity_class(ix, iy, iz)];
   return L.start[k] + ((int64_t)(iz >> 1) * L.nb[k][1] + (iy >> 1)) * L.nb[k][0] + (ix >> 1);
 }
 
@@ -93,12 +106,13 @@ struct Level {
 
 struct Ctx {
   int device = 0;
+  int order[kColours] = {7, 3, 4, 5, 2, 6, 1, 0};  // sweep order of new problems (kDefaultOrder)
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   std::vector<Level> levels;
   std::string err;
-  double *partials = nullptr;  // [4 * kReduceBlocks]
-  unsigned *ticket = nullptr;
+  double *partials = nullptr;  // per-block partial sums, 2 x the largest level-0 grid
+  int64_t reduce_cap = 0;      // largest grid the scratch covers
   double *scal = nullptr;      // device scalars [S_COUNT]
   double *hist = nullptr;      // device residual history
   int hist_cap = 0;
@@ -137,7 +151,9 @@ int mg_fp32(Ctx *c, int level, const float *r, float *x);
 int mg_fp16(Ctx *c, const float *r, float *x);
 int spmv_fp64(Ctx *c, int level, const double *x, double *y);
 int symgs_fp64(Ctx *c, int level, const double *r, double *x);
-int reduce_grid_size(int64_t n);
+int ensure_reduce(Ctx *c, int64_t grid);
+int finish(Ctx *c, int nv, int64_t grid, int op);
+inline unsigned chunk_blocks(int64_t n) { return (unsigned)((n + 4 * 256 - 1) / (4 * 256)); }
 
 }  // namespace hpcgx
 

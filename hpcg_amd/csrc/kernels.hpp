@@ -28,19 +28,31 @@ __device__ __forceinline__ int64_t tile_id() {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-// matrix loads: streamed once per pass, kept out of the way of the vectors
-__device__ __forceinline__ int ld_col(const int *p) { return __builtin_nontemporal_load(p); }
-template <class ACC>
+// Matrix loads.  NT (non-temporal): the fine level's matrix is streamed once
+// per pass and is far larger than any cache, so it is kept out of the way of
+// the vectors; coarse-level matrices fit in the 256 MB Infinity Cache and
+// are loaded normally so that the V-cycle's repeated passes can hit there.
+template <bool NT = true>
+__device__ __forceinline__ int ld_col(const int *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <class ACC, bool NT = true>
 __device__ __forceinline__ ACC ld_val(const double *p) {
-  return (ACC)__builtin_nontemporal_load(p);
+  if constexpr (NT) return (ACC)__builtin_nontemporal_load(p);
+  else return (ACC)*p;
 }
-template <class ACC>
+template <class ACC, bool NT = true>
 __device__ __forceinline__ ACC ld_val(const float *p) {
-  return (ACC)__builtin_nontemporal_load(p);
+  if constexpr (NT) return (ACC)__builtin_nontemporal_load(p);
+  else return (ACC)*p;
 }
-template <class ACC>
+template <class ACC, bool NT = true>
 __device__ __forceinline__ ACC ld_val(const __half *p) {
-  const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+  const unsigned short *q = reinterpret_cast<const unsigned short *>(p);
+  unsigned short bits;
+  if constexpr (NT) bits = __builtin_nontemporal_load(q);
+  else bits = *q;
   return (ACC)__half2float(__ushort_as_half(bits));
 }
 
@@ -53,14 +65,17 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Reduces NV values over the whole grid.  Returns true in thread 0 of the
-// block that finished last, with the grid totals in tot[].  Partials are
-// summed in block order by every launch, so results do not depend on timing.
+// Deterministic reductions in two kernels.  Each block of the producing
+// kernel reduces its values in a fixed tree (shuffles, then 4 waves in order)
+// and stores one partial per value with a plain store; a one-block finishing
+// kernel then sums the partials in a fixed order and applies the scalar
+// update.  Within a kernel no fences or atomics are needed: the kernel
+// boundary makes the partials visible (an agent-scope fence inside a kernel
+// writes back or invalidates the whole L2 of an XCD, which costs far more
+// than the extra launch).
 template <int NV>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, unsigned *ticket,
-                                            double (&tot)[NV]) {
+__device__ __forceinline__ void block_partials(const double (&v)[NV], double *partials) {
   __shared__ double sh[4][NV];
-  __shared__ int is_last;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
@@ -70,33 +85,82 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, u
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      const double s = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
-      __hip_atomic_store(&partials[(size_t)k * gridDim.x + blockIdx.x], s, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    is_last = (t == gridDim.x - 1);
+    for (int k = 0; k < NV; ++k)
+      partials[(size_t)k * gridDim.x + blockIdx.x] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// What the finishing kernel does with the totals (t0, t1).
+enum FinishOp : int {
+  F_DOT = 0,        // scal[S_DOT] = t0
+  F_RZ_FIRST,       // rtz = t0                                   (CG.cpp:110)
+  F_RZ,             // oldrtz = rtz; rtz = t0; beta = rtz/oldrtz  (CG.cpp:112-114)
+  F_PAP,            // pAp = t0; alpha = rtz/pAp                  (CG.cpp:119-120)
+  F_NORMR2,         // normr^2 = t0                               (CG.cpp:123)
+  F_FCG,            // oldrtz = rtz; rtz = t0; beta = -alpha t1 / oldrtz
+};
+
+constexpr int kFinishThreads = 1024;
+
+template <int NV>
+__global__ __launch_bounds__(kFinishThreads) void k_finish(const double *__restrict__ partials, int64_t G,
+                                                           double *scal, int op) {
+  __shared__ double sh[kFinishThreads / 64][NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock)
-      s += __hip_atomic_load(&partials[(size_t)k * gridDim.x + i], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    s = wave_sum(s);
+    const double *p = partials + (size_t)k * G;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * kFinishThreads < G; i += 4 * kFinishThreads) {
+      a0 += p[i];
+      a1 += p[i + kFinishThreads];
+      a2 += p[i + 2 * kFinishThreads];
+      a3 += p[i + 3 * kFinishThreads];
+    }
+    for (; i < G; i += kFinishThreads) a0 += p[i];
+    const double s = wave_sum((a0 + a1) + (a2 + a3));
     if (lane == 0) sh[w][k] = s;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return false;
+  if (threadIdx.x != 0) return;
+  double t[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) tot[k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
-  __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int j = 0; j < kFinishThreads / 64; ++j) s += sh[j][k];
+    t[k] = s;
+  }
+  switch (op) {
+    case F_DOT: scal[S_DOT] = t[0]; break;
+    case F_RZ_FIRST: scal[S_RTZ] = t[0]; break;
+    case F_RZ: {
+      const double old = scal[S_RTZ];
+      scal[S_OLDRTZ] = old;
+      scal[S_RTZ] = t[0];
+      scal[S_BETA] = t[0] / old;
+      break;
+    }
+    case F_PAP:
+      scal[S_PAP] = t[0];
+      scal[S_ALPHA] = scal[S_RTZ] / t[0];
+      break;
+    case F_NORMR2: scal[S_NORMR2] = t[0]; break;
+    case F_FCG: {
+      const double old = scal[S_RTZ];
+      scal[S_OLDRTZ] = old;
+      scal[S_RTZ] = t[0];
+      scal[S_BETA] = -scal[S_ALPHA] * t[NV - 1] / old;
+      break;
+    }
+  }
+}
+
+// Streaming reductions give each block one chunk of kChunk * kBlock
+// consecutive elements, kChunk per thread, each a coalesced wave access.
+constexpr int kChunk = 4;
+__device__ __forceinline__ int64_t chunk_index(int j) {
+  return ((int64_t)blockIdx.x * kChunk + j) * kBlock + threadIdx.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -122,13 +186,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ cols,
   y[row] = (XT)sum;
 }
 
-// Slots whose neighbour lies in a block swept before block k (block k holds
-// parity class 7-k; the neighbour at offset d has class c ^ oddness(d)).
+// Slots whose neighbour lies in a block swept before block k in the default
+// order (the neighbour at offset d has class c ^ oddness(d)).
 __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
+  int pos[kColours] = {};
+  for (int j = 0; j < kColours; ++j) pos[kDefaultOrder[j]] = j;
   uint32_t m = 0;
   for (int s = 0; s < kSlots; ++s) {
     const int odd = ((s % 3) != 1) | ((((s / 3) % 3) != 1) << 1) | (((s / 9) != 1) << 2);
-    if (7 - ((7 - k) ^ odd) < k) m |= 1u << s;
+    if (pos[kDefaultOrder[k] ^ odd] < k) m |= 1u << s;
   }
   return m;
 }
@@ -139,7 +205,9 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
 // neighbours in blocks swept before ZK can be nonzero; the other terms are
 // exact zeros and are not read, and neither is x_i.  The skipped set is a
 // compile-time constant, so the remaining loads issue back to back.
-template <class VT, class XT, class ACC, int ZK>
+// FINE: the finest level (streamed matrix loads; its own instantiation, so
+// profiles show the dominant launches separately from the coarse levels').
+template <class VT, class XT, class ACC, int ZK, bool FINE>
 __global__ __launch_bounds__(kBlock) void k_gs(const int *__restrict__ cols,
                                                const VT *__restrict__ vals,
                                                const XT *__restrict__ r, XT *__restrict__ x,
@@ -149,13 +217,13 @@ __global__ __launch_bounds__(kBlock) void k_gs(const int *__restrict__ cols,
   if (t >= nrows) return;
   const int64_t row = row0 + t;
   const size_t b = ell_base(row);
-  const ACC d = ld_val<ACC>(vals + b + kDiagSlot * kSlice);
+  const ACC d = ld_val<ACC, FINE>(vals + b + kDiagSlot * kSlice);
   ACC sum = (ACC)r[row];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) {
     if (!((mask >> s) & 1u)) continue;
-    const int c = ld_col(cols + b + s * kSlice);
-    const ACC v = ld_val<ACC>(vals + b + s * kSlice);
+    const int c = ld_col<FINE>(cols + b + s * kSlice);
+    const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
     sum -= v * (ACC)x[c];
   }
   if (ZK < 0) sum += (ACC)x[row] * d;

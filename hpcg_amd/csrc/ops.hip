@@ -21,28 +21,51 @@ const __half *vals_of<__half>(const Level &L) { return L.vals16; }
 // from exactly the neighbour values the forward step used, so it is elided.
 // zero_start: x is known to be zero on entry (the V-cycle's first sweep), so
 // x is not read before it is written.
-template <class VT, class XT, class ACC, int ZK>
+template <class VT, class XT, class ACC, int ZK, bool FINE>
 static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, int k) {
   const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
-  if (nr > 0) k_gs<VT, XT, ACC, ZK><<<tiles(nr), kBlock, 0, c->stream>>>(L.cols, v, r, x, r0, nr);
+  if (nr > 0) k_gs<VT, XT, ACC, ZK, FINE><<<tiles(nr), kBlock, 0, c->stream>>>(L.cols, v, r, x, r0, nr);
+}
+
+template <class XT>
+__global__ __launch_bounds__(kBlock) void k_zero(XT *__restrict__ x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = (XT)0;
+}
+
+static bool default_order(const Level &L) {
+  for (int k = 0; k < kColours; ++k)
+    if (L.lay.cls[k] != kDefaultOrder[k]) return false;
+  return true;
+}
+
+template <class VT, class XT, class ACC, bool FINE>
+static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start) {
+  const VT *v = vals_of<VT>(L);
+  if (zero_start && !default_order(L)) {  // other orders: zero x, then full steps
+    k_zero<XT><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(x, L.lay.npad);
+    zero_start = false;
+  }
+  if (zero_start) {
+    gs_block<VT, XT, ACC, 0, FINE>(c, L, v, r, x, 0);
+    gs_block<VT, XT, ACC, 1, FINE>(c, L, v, r, x, 1);
+    gs_block<VT, XT, ACC, 2, FINE>(c, L, v, r, x, 2);
+    gs_block<VT, XT, ACC, 3, FINE>(c, L, v, r, x, 3);
+    gs_block<VT, XT, ACC, 4, FINE>(c, L, v, r, x, 4);
+    gs_block<VT, XT, ACC, 5, FINE>(c, L, v, r, x, 5);
+    gs_block<VT, XT, ACC, 6, FINE>(c, L, v, r, x, 6);
+    gs_block<VT, XT, ACC, 7, FINE>(c, L, v, r, x, 7);
+  } else {
+    for (int k = 0; k < kColours; ++k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
+  }
+  for (int k = kColours - 2; k >= 0; --k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
 }
 
 template <class VT, class XT, class ACC>
-static int symgs_launch(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start) {
-  const VT *v = vals_of<VT>(L);
-  if (zero_start) {
-    gs_block<VT, XT, ACC, 0>(c, L, v, r, x, 0);
-    gs_block<VT, XT, ACC, 1>(c, L, v, r, x, 1);
-    gs_block<VT, XT, ACC, 2>(c, L, v, r, x, 2);
-    gs_block<VT, XT, ACC, 3>(c, L, v, r, x, 3);
-    gs_block<VT, XT, ACC, 4>(c, L, v, r, x, 4);
-    gs_block<VT, XT, ACC, 5>(c, L, v, r, x, 5);
-    gs_block<VT, XT, ACC, 6>(c, L, v, r, x, 6);
-    gs_block<VT, XT, ACC, 7>(c, L, v, r, x, 7);
-  } else {
-    for (int k = 0; k < kColours; ++k) gs_block<VT, XT, ACC, -1>(c, L, v, r, x, k);
-  }
-  for (int k = kColours - 2; k >= 0; --k) gs_block<VT, XT, ACC, -1>(c, L, v, r, x, k);
+static int symgs_launch(Ctx *c, int l, const XT *r, XT *x, bool zero_start) {
+  const Level &L = c->levels[l];
+  if (l == 0) sweep<VT, XT, ACC, true>(c, L, r, x, zero_start);
+  else sweep<VT, XT, ACC, false>(c, L, r, x, zero_start);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -52,7 +75,7 @@ static int symgs_launch(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_st
 template <class VT, class XT, class ACC, bool SCALE>
 static int vcycle(Ctx *c, int l, const XT *r, XT *x) {
   const Level &L = c->levels[l];
-  int rc = symgs_launch<VT, XT, ACC>(c, L, r, x, true);
+  int rc = symgs_launch<VT, XT, ACC>(c, l, r, x, true);
   if (rc || l + 1 >= (int)c->levels.size()) return rc;
   Level &C = c->levels[l + 1];
   XT *rc_v, *xc_v;
@@ -63,14 +86,14 @@ static int vcycle(Ctx *c, int l, const XT *r, XT *x) {
     rc_v = reinterpret_cast<XT *>(C.r32);
     xc_v = reinterpret_cast<XT *>(C.x32);
   }
-  const int64_t nc = L.lay.count[7], r0 = L.lay.start[7];
+  const int64_t nc = L.lay.count[L.lay.inj], r0 = L.lay.start[L.lay.inj];
   k_restrict<VT, XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(L.cols, vals_of<VT>(L), r, x, rc_v,
                                                                        L.inj, L.sigma, r0, nc);
   HPCGX_CHECK_LAUNCH(c);
   if ((rc = vcycle<VT, XT, ACC, SCALE>(c, l + 1, rc_v, xc_v))) return rc;
   k_prolong<XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc_v, L.inj, L.sigma, r0, nc);
   HPCGX_CHECK_LAUNCH(c);
-  return symgs_launch<VT, XT, ACC>(c, L, r, x, false);
+  return symgs_launch<VT, XT, ACC>(c, l, r, x, false);
 }
 
 // vector kernels
@@ -86,32 +109,47 @@ __global__ __launch_bounds__(kBlock) void k_waxpby(int64_t n, double alpha,
 
 // out = x . y (ComputeDotProduct_ref.cpp:54-64, fixed-order tree)
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double *__restrict__ x,
-                                                const double *__restrict__ y, double *partials,
-                                                unsigned *ticket, double *out) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
+                                                const double *__restrict__ y, double *partials) {
+  double a[kChunk], b[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    a[j] = i < n ? x[i] : 0.0;
+    b[j] = i < n ? y[i] : 0.0;
+  }
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-    acc += x[i] * y[i];
-  double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, ticket, tot)) out[0] = tot[0];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) acc += a[j] * b[j];
+  const double v[1] = {acc};
+  block_partials<1>(v, partials);
 }
 
 // w = alpha x + beta y ; out = w . w   (WAXPBY then DOT of CG.cpp:88-89 / 122-123)
 __global__ __launch_bounds__(kBlock) void k_waxpby_dot(int64_t n, double alpha,
                                                        const double *__restrict__ x, double beta,
                                                        const double *__restrict__ y,
-                                                       double *__restrict__ w, double *partials,
-                                                       unsigned *ticket, double *out) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const double v = waxpby1(alpha, x[i], beta, y[i]);
-    w[i] = v;
-    acc += v * v;
+                                                       double *__restrict__ w, double *partials) {
+  double a[kChunk], b[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    a[j] = i < n ? x[i] : 0.0;
+    b[j] = i < n ? y[i] : 0.0;
   }
-  double v[1] = {acc}, tot[1];
-  if (grid_reduce<1>(v, partials, ticket, tot)) out[0] = tot[0];
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    const double v = waxpby1(alpha, a[j], beta, b[j]);
+    if (i < n) {
+      w[i] = v;
+      acc += v * v;
+    }
+  }
+  const double v[1] = {acc};
+  block_partials<1>(v, partials);
 }
+
 
 int mg_fp64(Ctx *c, int level, const double *r, double *x) {
   return vcycle<double, double, double, false>(c, level, r, x);
@@ -162,7 +200,7 @@ int spmv_fp64(Ctx *c, int level, const double *x, double *y) {
 }
 
 int symgs_fp64(Ctx *c, int level, const double *r, double *x) {
-  return symgs_launch<double, double, double>(c, c->levels[level], r, x, false);
+  return symgs_launch<double, double, double>(c, level, r, x, false);
 }
 
 }  // namespace hpcgx
@@ -198,9 +236,9 @@ int hpcgx_restrict(hpcgx_ctx *c, int level, const double *r, const double *x, do
   if (rc) return rc;
   if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "restrict: level has no coarse level");
   const Level &L = c->levels[level];
-  const int64_t nc = L.lay.count[7];
+  const int64_t nc = L.lay.count[L.lay.inj];
   k_restrict<double, double, double, false><<<tiles(nc), kBlock, 0, c->stream>>>(
-      L.cols, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[7], nc);
+      L.cols, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[L.lay.inj], nc);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -210,9 +248,9 @@ int hpcgx_prolong(hpcgx_ctx *c, int level, const double *xc, double *x) {
   if (rc) return rc;
   if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "prolong: level has no coarse level");
   const Level &L = c->levels[level];
-  const int64_t nc = L.lay.count[7];
+  const int64_t nc = L.lay.count[L.lay.inj];
   k_prolong<double, double, false><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc, L.inj, nullptr,
-                                                                        L.lay.start[7], nc);
+                                                                        L.lay.start[L.lay.inj], nc);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -259,8 +297,9 @@ int hpcgx_dot(hpcgx_ctx *c, int64_t n, const double *x, const double *y, double 
   if (rc) return rc;
   if (!x || !y || !result) return set_err(c, HPCGX_EINVAL, "dot: null argument");
   const int64_t m = span(c, n);
-  k_dot<<<reduce_grid_size(m), kBlock, 0, c->stream>>>(m, x, y, c->partials, c->ticket, c->scal + S_DOT);
+  k_dot<<<chunk_blocks(m), kBlock, 0, c->stream>>>(m, x, y, c->partials);
   HPCGX_CHECK_LAUNCH(c);
+  if ((rc = finish(c, 1, chunk_blocks(m), F_DOT))) return rc;
   HPCGX_HIP(c, hipMemcpyAsync(c->hbuf, c->scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   *result = c->hbuf[0];
@@ -284,9 +323,9 @@ int hpcgx_waxpby_dot(hpcgx_ctx *c, int64_t n, double alpha, const double *x, dou
   if (rc) return rc;
   if (!x || !y || !w || !result) return set_err(c, HPCGX_EINVAL, "waxpby_dot: null argument");
   const int64_t m = span(c, n);
-  k_waxpby_dot<<<reduce_grid_size(m), kBlock, 0, c->stream>>>(m, alpha, x, beta, y, w, c->partials, c->ticket,
-                                                              c->scal + S_DOT);
+  k_waxpby_dot<<<chunk_blocks(m), kBlock, 0, c->stream>>>(m, alpha, x, beta, y, w, c->partials);
   HPCGX_CHECK_LAUNCH(c);
+  if ((rc = finish(c, 1, chunk_blocks(m), F_DOT))) return rc;
   HPCGX_HIP(c, hipMemcpyAsync(c->hbuf, c->scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   *result = c->hbuf[0];

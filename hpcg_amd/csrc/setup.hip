@@ -59,7 +59,7 @@ static void parallel_for(int64_t n, F f) {
   for (auto &t : th) t.join();
 }
 
-Layout make_layout(int nx, int ny, int nz) {
+Layout make_layout(int nx, int ny, int nz, const int *order) {
   Layout L;
   L.nx = nx;
   L.ny = ny;
@@ -67,7 +67,12 @@ Layout make_layout(int nx, int ny, int nz) {
   L.n = (int64_t)nx * ny * nz;
   L.start[0] = 0;
   for (int k = 0; k < kColours; ++k) {
-    const int c = 7 - k;
+    L.cls[k] = order[k];
+    L.pos[order[k]] = k;
+  }
+  L.inj = L.pos[0];
+  for (int k = 0; k < kColours; ++k) {
+    const int c = L.cls[k];
     L.nb[k][0] = (nx + 1 - (c & 1)) / 2;
     L.nb[k][1] = (ny + 1 - ((c >> 1) & 1)) / 2;
     L.nb[k][2] = (nz + 1 - ((c >> 2) & 1)) / 2;
@@ -85,14 +90,34 @@ static LayoutDev to_dev(const Layout &L) {
   d.ny = L.ny;
   d.nz = L.nz;
   for (int k = 0; k <= kColours; ++k) d.start[k] = L.start[k];
-  for (int k = 0; k < kColours; ++k)
+  for (int k = 0; k < kColours; ++k) {
     for (int a = 0; a < 3; ++a) d.nb[k][a] = L.nb[k][a];
+    d.cls[k] = L.cls[k];
+    d.pos[k] = L.pos[k];
+  }
   return d;
 }
 
-int reduce_grid_size(int64_t n) {
-  const int64_t tiles = (n + kBlock - 1) / kBlock;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kReduceBlocks));
+// Partial-sum scratch for reductions over grids of up to `grid` blocks.
+int ensure_reduce(Ctx *c, int64_t grid) {
+  if (c->reduce_cap >= grid) return HPCGX_OK;
+  if (c->partials) dfree(c, c->partials, 0);
+  c->partials = nullptr;
+  c->reduce_cap = 0;
+  int rc = dalloc(c, (void **)&c->partials, (size_t)(2 * grid + 16) * sizeof(double), false);
+  if (rc) return rc;
+  c->reduce_cap = grid;
+  return HPCGX_OK;
+}
+
+// One-block finishing kernel of a reduction over `grid` producer blocks.
+int finish(Ctx *c, int nv, int64_t grid, int op) {
+  if (nv == 1)
+    k_finish<1><<<1, kFinishThreads, 0, c->stream>>>(c->partials, grid, c->scal, op);
+  else
+    k_finish<2><<<1, kFinishThreads, 0, c->stream>>>(c->partials, grid, c->scal, op);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
 }
 
 int ensure_stage(Ctx *c, int64_t n) {
@@ -113,10 +138,10 @@ static void free_level(Ctx *c, Level &L) {
   dfree(c, L.vals32, m * sizeof(float));
   dfree(c, L.vals16, m * sizeof(__half));
   dfree(c, L.dinv, L.lay.npad * sizeof(float));
-  dfree(c, L.sigma, L.lay.count[7] * sizeof(float));
+  dfree(c, L.sigma, L.lay.count[L.lay.inj] * sizeof(float));
   dfree(c, L.perm, L.lay.n * sizeof(int));
   dfree(c, L.iperm, L.lay.npad * sizeof(int));
-  dfree(c, L.inj, L.lay.count[7] * sizeof(int));
+  dfree(c, L.inj, L.lay.count[L.lay.inj] * sizeof(int));
   dfree(c, L.r, L.lay.npad * sizeof(double));
   dfree(c, L.x, L.lay.npad * sizeof(double));
   dfree(c, L.r32, L.lay.npad * sizeof(float));
@@ -186,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, 
     }
     return;
   }
-  const int c = 7 - k;
+  const int c = L.cls[k];
   const int ix = 2 * (int)(idx % nbx) + (c & 1);
   const int iy = 2 * (int)((idx / nbx) % nby) + ((c >> 1) & 1);
   const int iz = 2 * (int)(idx / ((int64_t)nbx * nby)) + ((c >> 2) & 1);
@@ -253,7 +278,7 @@ static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, 
   c->prepared = 0;
   c->levels.emplace_back();
   Level &L = c->levels.back();
-  L.lay = make_layout(nx, ny, nz);
+  L.lay = make_layout(nx, ny, nz, c->order);
   L.ldev = to_dev(L.lay);
   const Layout &lay = L.lay;
   const int64_t n = lay.n, npad = lay.npad;
@@ -261,7 +286,7 @@ static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, 
   L.perm_h.assign(n, 0);
   L.iperm_h.assign(npad, -1);
   for (int k = 0; k < kColours; ++k) {
-    const int cl = 7 - k;
+    const int cl = lay.cls[k];
     for (int64_t idx = 0; idx < lay.count[k]; ++idx) {
       const int ix = 2 * (int)(idx % lay.nb[k][0]) + (cl & 1);
       const int iy = 2 * (int)((idx / lay.nb[k][0]) % lay.nb[k][1]) + ((cl >> 1) & 1);
@@ -351,7 +376,8 @@ int ensure_ready(Ctx *c) {
     Level &F = c->levels[l], &Cc = c->levels[l + 1];
     const int64_t nc = Cc.lay.n;
     if (F.inj) continue;
-    if (F.lay.count[7] != nc || F.lay.nx != 2 * Cc.lay.nx || F.lay.ny != 2 * Cc.lay.ny ||
+    const int ib = F.lay.inj;
+    if (F.lay.count[ib] != nc || F.lay.nx != 2 * Cc.lay.nx || F.lay.ny != 2 * Cc.lay.ny ||
         F.lay.nz != 2 * Cc.lay.nz)
       return set_err(c, HPCGX_ESTATE, "level " + std::to_string(l + 1) + " is not the 2x coarsening of level " + std::to_string(l));
     if ((int64_t)F.f2c_h.size() != nc)
@@ -361,12 +387,12 @@ int ensure_ready(Ctx *c) {
       const int f = F.f2c_h[k];
       if (f < 0 || f >= F.lay.n) return set_err(c, HPCGX_EINVAL, "f2c out of range");
       const int pf = F.perm_h[f];
-      if (pf < F.lay.start[7] || pf >= F.lay.start[7] + F.lay.count[7] || inv[pf] >= 0)
+      if (pf < F.lay.start[ib] || pf >= F.lay.start[ib] + F.lay.count[ib] || inv[pf] >= 0)
         return set_err(c, HPCGX_EINVAL, "f2c must map coarse rows one-to-one onto fine rows with even coordinates");
       inv[pf] = (int)k;
     }
     std::vector<int> inj(nc);
-    for (int64_t t = 0; t < nc; ++t) inj[t] = Cc.perm_h[inv[F.lay.start[7] + t]];
+    for (int64_t t = 0; t < nc; ++t) inj[t] = Cc.perm_h[inv[F.lay.start[ib] + t]];
     if ((rc = dalloc(c, (void **)&F.inj, nc * sizeof(int), false))) return rc;
     HPCGX_HIP(c, hipMemcpy(F.inj, inj.data(), nc * sizeof(int), hipMemcpyHostToDevice));
   }
@@ -376,6 +402,7 @@ int ensure_ready(Ctx *c) {
     if (!L.x && (rc = dalloc(c, (void **)&L.x, L.lay.npad * sizeof(double)))) return rc;
   }
   const int64_t np = c->levels[0].lay.npad;
+  if ((rc = ensure_reduce(c, std::max<int64_t>((np + kBlock - 1) / kBlock, chunk_blocks(np))))) return rc;
   if (!c->cg_r) {
     if ((rc = dalloc(c, (void **)&c->cg_r, np * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void **)&c->cg_z, np * sizeof(double)))) return rc;
@@ -457,10 +484,10 @@ static int build_precision(Ctx *c, int prec) {
   if (prec == HPCGX_PREC_FP16) {
     for (int l = 0; l + 1 < nl; ++l) {
       Level &F = c->levels[l];
-      const int64_t nc = F.lay.count[7];
+      const int64_t nc = F.lay.count[F.lay.inj];
       if (!F.sigma && (rc = dalloc(c, (void **)&F.sigma, nc * sizeof(float), false))) return rc;
       k_sigma<<<nblk(nc), kBlock, 0, c->stream>>>(F.vals, c->levels[l + 1].vals, F.inj, F.sigma,
-                                                  F.lay.start[7], nc);
+                                                  F.lay.start[F.lay.inj], nc);
       HPCGX_CHECK_LAUNCH(c);
     }
     Level &L0 = c->levels[0];
@@ -511,8 +538,7 @@ int hpcgx_create(int device, hpcgx_ctx **out) {
   }
   c->stream = c->own_stream;
   int rc;
-  if ((rc = dalloc(c, (void **)&c->partials, 4 * kReduceBlocks * sizeof(double))) ||
-      (rc = dalloc(c, (void **)&c->ticket, 64 * sizeof(unsigned))) ||
+  if ((rc = ensure_reduce(c, 4096)) ||
       (rc = dalloc(c, (void **)&c->scal, S_COUNT * sizeof(double)))) {
     hpcgx_destroy(c);
     return rc;
@@ -544,7 +570,6 @@ int hpcgx_destroy(hpcgx_ctx *c) {
   (void)hipFree(c->cg_z32);
   (void)hipFree(c->stage);
   (void)hipFree(c->partials);
-  (void)hipFree(c->ticket);
   (void)hipFree(c->scal);
   (void)hipFree(c->hist);
   if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -601,7 +626,7 @@ int hpcgx_problem_generate(hpcgx_ctx *c, int nx, int ny, int nz, int levels, con
   for (int l = 0; l < levels; ++l) {
     c->levels.emplace_back();
     Level &L = c->levels.back();
-    L.lay = make_layout(nx >> l, ny >> l, nz >> l);
+    L.lay = make_layout(nx >> l, ny >> l, nz >> l, c->order);
     L.ldev = to_dev(L.lay);
     L.kappa = kappa != nullptr;
     if (L.lay.npad >= ((int64_t)1 << 31)) return set_err(c, HPCGX_EINVAL, "problem_generate: too many rows for 32-bit columns");
@@ -644,6 +669,18 @@ int hpcgx_problem_generate(hpcgx_ctx *c, int nx, int ny, int nz, int levels, con
   }
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   return ensure_ready(c);
+}
+
+int hpcgx_set_colour_order(hpcgx_ctx *c, const int *order) {
+  if (!c || !order) return HPCGX_EINVAL;
+  int seen = 0;
+  for (int k = 0; k < kColours; ++k) {
+    if (order[k] < 0 || order[k] >= kColours || (seen >> order[k]) & 1)
+      return set_err(c, HPCGX_EINVAL, "set_colour_order: not a permutation of the 8 parity classes");
+    seen |= 1 << order[k];
+  }
+  for (int k = 0; k < kColours; ++k) c->order[k] = order[k];
+  return HPCGX_OK;
 }
 
 int hpcgx_level_info(const hpcgx_ctx *c, int level, int64_t *out) {

@@ -107,3 +107,19 @@ def algorithmic_bytes_per_iteration(nx, ny, nz, levels=4, v_mg=8, s_mg=8, fused=
     else:
         total += 3 * 16 * n0 + 3 * 24 * n0
     return total
+
+
+def hpcg_rhs(nx, ny, nz):
+    """b_i = 26 - (nnz_i - 1) in natural order (GenerateProblem_ref.cpp:178),
+    nnz_i the product of in-grid neighbour counts per dimension."""
+    import numpy as np
+
+    def cnt(n):
+        c = np.full(n, 3.0)
+        c[0] -= 1
+        c[-1] -= 1
+        if n == 1:
+            c[0] = 1.0
+        return c
+    nnz = cnt(nz)[:, None, None] * cnt(ny)[None, :, None] * cnt(nx)[None, None, :]
+    return (26.0 - (nnz - 1.0)).reshape(-1)

@@ -65,6 +65,12 @@ int hpcgx_sync(hpcgx_ctx *ctx);
 
 /* ---- problem (OptimizeProblem.cpp:37-100) ----------------------------- */
 
+/* Sweep order of the problems uploaded or generated afterwards: order[k] is the
+ * parity class ((x&1)|(y&1)<<1|(z&1)<<2) of the k-th colour of the forward
+ * sweep (the backward sweep reverses it).  Default 7,3,4,5,2,6,1,0.  The
+ * reference's colouring example (OptimizeProblem.cpp:42-97) fixes the colour
+ * classes; this picks their order. */
+int hpcgx_set_colour_order(hpcgx_ctx *ctx, const int *order);
 /* Uploads one level in the reference's row-pointer form (SparseMatrix.hpp:47-51:
  * nonzerosInRow, mtxIndL, matrixValues) of an nx*ny*nz grid, levels from fine
  * (0) to coarse.  f2c: the level's f2cOperator (MGData.hpp:31), length = rows

@@ -120,8 +120,11 @@ orc_level *orc_create(int nx, int ny, int nz, int levels, const double *kappa) {
     memcpy(L->kappa, kappa, n * sizeof(double));
   }
   generate_level(L);
-  /* default sweep order: parity classes 7,6,...,0 (the device's order) */
-  for (int k = 0; k < 8; ++k) L->order[k] = 7 - k;
+  /* default sweep order: the device's (parity classes 7,3,4,5,2,6,1,0) */
+  {
+    static const int dflt[8] = {7, 3, 4, 5, 2, 6, 1, 0};
+    for (int k = 0; k < 8; ++k) L->order[k] = dflt[k];
+  }
   build_colour_lists(L);
   L->Axf = (double *)calloc(n, sizeof(double));
   L->Axf32 = (float *)calloc(n, sizeof(float));

@@ -64,7 +64,7 @@ void orc_set_diagonal(orc_level *L, const double *d);
 
 /* Colouring.  parity class p = (x&1) | (y&1)<<1 | (z&1)<<2 ; order[k] is the
  * parity class swept k-th in the forward sweep (backward sweep reverses it).
- * Applied to every level of the hierarchy.  Default: 7,6,5,4,3,2,1,0. */
+ * Applied to every level of the hierarchy.  Default: 7,3,4,5,2,6,1,0. */
 void orc_set_colour_order(orc_level *L, const int order[8]);
 void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours);
 

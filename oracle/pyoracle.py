@@ -13,6 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libhpcg_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libhpcg_ref.so")
+REF_OMP_SO = os.path.join(HERE, "_ref", "libhpcg_ref_omp.so")
 REF_BIN = os.path.join(HERE, "_ref", "xhpcg_ref")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -22,7 +23,7 @@ _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 _lib = None
-_ref = None
+_ref = {}
 
 
 def lib():
@@ -249,16 +250,18 @@ def f16_bits_to_f32(h):
 # --------------------------------------------------------------------------
 # the reference's own code (oracle/_ref/libhpcg_ref.so)
 
-def ref_available():
-    return os.path.exists(REF_SO)
+def ref_available(omp=False):
+    return os.path.exists(REF_OMP_SO if omp else REF_SO)
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        if not ref_available():
-            raise RuntimeError("reference parity build missing: run `make -C oracle ref`")
-        R = C.CDLL(REF_SO)
+def ref(omp=False):
+    """The reference's own code: the serial no-FMA parity build, or (omp) the
+    OpenMP build with the flags of setup/Make.GCC_OMP."""
+    if omp not in _ref:
+        path = REF_OMP_SO if omp else REF_SO
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference build missing ({path}): run `make -C oracle ref`")
+        R = C.CDLL(path)
         vp = C.c_void_p
         R.ref_create.restype = vp
         R.ref_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
@@ -280,27 +283,28 @@ def ref():
         R.ref_cg.argtypes = [vp, vp, _dp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
                              C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
         R.ref_timer.restype = C.c_double
-        _ref = R
-    return _ref
+        _ref[omp] = R
+    return _ref[omp]
 
 
 class RefProblem:
     """The reference's own problem (GenerateProblem + 3 coarse levels) and _ref kernels."""
 
-    def __init__(self, nx, ny, nz, levels=4):
-        self.h = ref().ref_create(nx, ny, nz, levels)
+    def __init__(self, nx, ny, nz, levels=4, omp=False):
+        self.omp = omp
+        self.h = ref(omp).ref_create(nx, ny, nz, levels)
         self.levels = levels
         self.info = []
         for l in range(levels):
             out = np.zeros(6, np.int64)
-            ref().ref_level_info(self.h, l, out)
+            ref(omp).ref_level_info(self.h, l, out)
             self.info.append(dict(nrow=int(out[0]), ncol=int(out[1]), nnz=int(out[2]),
                                   dims=(int(out[3]), int(out[4]), int(out[5]))))
         self.n = self.info[0]["nrow"]
 
     def __del__(self):
         try:
-            ref().ref_destroy(self.h)
+            ref(self.omp).ref_destroy(self.h)
         except Exception:
             pass
 
@@ -315,30 +319,30 @@ class RefProblem:
         if level + 1 < self.levels:
             f2c = np.zeros(self.info[level + 1]["nrow"], np.int32)
             ptr = f2c.ctypes.data_as(C.c_void_p)
-        ref().ref_level_export(self.h, level, nnz_row, cols, vals, diag, ptr)
+        ref(self.omp).ref_level_export(self.h, level, nnz_row, cols, vals, diag, ptr)
         return nnz_row, cols.reshape(n, 27), vals.reshape(n, 27), diag, f2c
 
     def rhs(self):
         b, x, xe = np.empty(self.n), np.empty(self.n), np.empty(self.n)
-        ref().ref_rhs(self.h, b, x, xe)
+        ref(self.omp).ref_rhs(self.h, b, x, xe)
         return b, x, xe
 
     def set_diagonal(self, level, d):
-        ref().ref_set_diagonal(self.h, level, np.ascontiguousarray(d, np.float64))
+        ref(self.omp).ref_set_diagonal(self.h, level, np.ascontiguousarray(d, np.float64))
 
     def spmv(self, x, level=0):
         y = np.empty(self.info[level]["nrow"])
-        ref().ref_spmv(self.h, level, np.array(x, np.float64), y)
+        ref(self.omp).ref_spmv(self.h, level, np.array(x, np.float64), y)
         return y
 
     def symgs(self, r, x, level=0):
         x = np.array(x, np.float64, copy=True)
-        ref().ref_symgs(self.h, level, np.array(r, np.float64), x)
+        ref(self.omp).ref_symgs(self.h, level, np.array(r, np.float64), x)
         return x
 
     def mg(self, r, level=0):
         x = np.zeros(self.info[level]["ncol"])
-        ref().ref_mg(self.h, level, np.array(r, np.float64), x)
+        ref(self.omp).ref_mg(self.h, level, np.array(r, np.float64), x)
         return x
 
     def cg(self, x0=None, max_iter=50, tol=0.0, precond=True, b=None):
@@ -347,7 +351,7 @@ class RefProblem:
         times = np.zeros(10)
         bb = None if b is None else np.ascontiguousarray(b, np.float64)
         bp = None if bb is None else bb.ctypes.data_as(C.c_void_p)
-        ref().ref_cg(self.h, bp, x, max_iter, tol, int(precond), C.byref(it), C.byref(nr),
+        ref(self.omp).ref_cg(self.h, bp, x, max_iter, tol, int(precond), C.byref(it), C.byref(nr),
                      C.byref(nr0), times.ctypes.data_as(C.c_void_p))
         return dict(x=x, niters=it.value, normr=nr.value, normr0=nr0.value, times=times)
 

@@ -126,7 +126,11 @@ def test_testnorms(hip):
         ctx.fill(dx, 0.0)
         out = ctx.cg(db, dx, max_iter=50, tol=0.0)
         vals.append(out["normr"] / out["normr0"])
-    assert np.var(vals) == 0.0
+    assert len(set(vals)) == 1   # bit-identical: fixed-order reductions
+    # TestNorms.cpp:34-42 arithmetic: mean via deltas from the first sample
+    mean = vals[0] + sum(v - vals[0] for v in vals) / len(vals)
+    variance = sum((v - mean) ** 2 for v in vals) / len(vals)
+    assert variance < 1.0e-6
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
