@@ -69,9 +69,9 @@ def build_dropin(verbose=False):
     if not srcs:
         return None
     tmp = DROPIN_LIB + ".tmp"
-    _run(["g++", "-O2", "-std=c++11", "-fPIC", "-shared", "-DHPCG_NO_MPI", "-DHPCG_NO_OPENMP",
+    _run(["g++", "-O2", "-std=c++11", "-fPIC", "-shared", "-Wall", "-DHPCG_NO_MPI",
           f"-I{DROPIN}", f"-I{INCLUDE}", "-o", tmp, *[os.path.join(DROPIN, s) for s in srcs],
-          f"-L{PKG}", "-lhpcgx", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"])
+          f"-L{PKG}", "-lhpcgx", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, DROPIN_LIB)
     if verbose:
         print("built", DROPIN_LIB)

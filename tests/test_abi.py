@@ -51,3 +51,30 @@ def test_flop_model_matches_reference_closed_form():
     f = hpcg_flops(sh, iters=50, sets=1)
     assert f["total"] == pytest.approx(50 * per_it + 4 * n + 2 * m, rel=1e-15)
     assert per_it / 1e9 == pytest.approx(2.618, abs=5e-4)
+
+
+def test_dropin_library_exports_reference_hooks():
+    """The C++ hooks keep the reference signatures (CG.hpp:22-24,
+    ComputeSPMV.hpp:20, ComputeSYMGS.hpp:20, ComputeMG.hpp:20,
+    ComputeDotProduct.hpp:18-19, ComputeWAXPBY.hpp:18-19,
+    OptimizeProblem.hpp:22,28, ExchangeHalo.hpp:19)."""
+    so = os.path.join(ROOT, "hpcg_amd", "libhpcg_dropin.so")
+    if not os.path.exists(so):
+        pytest.skip("drop-in library not built")
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    want = [
+        "CG(SparseMatrix_STRUCT const&, CGData_STRUCT&, Vector_STRUCT const&, Vector_STRUCT&, int, double, "
+        "int&, double&, double&, double*, bool)",
+        "ComputeSPMV(SparseMatrix_STRUCT const&, Vector_STRUCT&, Vector_STRUCT&)",
+        "ComputeSYMGS(SparseMatrix_STRUCT const&, Vector_STRUCT const&, Vector_STRUCT&)",
+        "ComputeMG(SparseMatrix_STRUCT const&, Vector_STRUCT const&, Vector_STRUCT&)",
+        "ComputeDotProduct(int, Vector_STRUCT const&, Vector_STRUCT const&, double&, double&, bool&)",
+        "ComputeWAXPBY(int, double, Vector_STRUCT const&, double, Vector_STRUCT const&, Vector_STRUCT&, bool&)",
+        "OptimizeProblem(SparseMatrix_STRUCT&, CGData_STRUCT&, Vector_STRUCT&, Vector_STRUCT&, Vector_STRUCT&)",
+        "OptimizeProblemMemoryUse(SparseMatrix_STRUCT const&)",
+        "ExchangeHalo(SparseMatrix_STRUCT const&, Vector_STRUCT&)",
+        "HPCG_VectorHostModified", "HPCG_VectorHostZeroed", "HPCG_VectorRelease",
+        "HPCG_MatrixDiagonalReplaced", "HPCG_MatrixRelease",
+    ]
+    for w in want:
+        assert w in out, w

@@ -1,0 +1,45 @@
+"""The drop-in boundary end to end: the reference's own driver and validation
+(main.cpp, TestCG, TestSymmetry, TestNorms, ReportResults) built against the
+drop-in headers and linked to the drop-in hooks over the HIP library
+(oracle/_ref/xhpcg_mi355x, made by `make -C oracle ref` from the reference
+sources).  The run must be VALID with every hook reported as optimised."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BINARY = os.path.join(ROOT, "oracle", "_ref", "xhpcg_mi355x")
+
+pytestmark = pytest.mark.gpu
+
+
+def run_driver(tmp_path, n):
+    if not os.path.exists(BINARY):
+        pytest.skip("reference driver not built (needs the reference sources: make -C oracle ref)")
+    env = dict(os.environ, OMP_NUM_THREADS="8")
+    p = subprocess.run([BINARY, str(n), str(n), str(n), "--rt=0"], cwd=tmp_path, env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    reports = [f for f in os.listdir(tmp_path) if f.startswith("HPCG-Benchmark")]
+    assert len(reports) == 1, (os.listdir(tmp_path), p.stderr[-2000:])
+    text = open(os.path.join(tmp_path, reports[0])).read()
+    kv = {}
+    for line in text.splitlines():
+        if "=" in line:
+            k, _, v = line.rpartition("=")
+            kv[k] = v
+    return text, kv
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_reference_driver_valid_with_dropin_hooks(tmp_path, n):
+    text, kv = run_driver(tmp_path, n)
+    assert "HPCG result is VALID" in text, text[-3000:]
+    assert kv["Spectral Convergence Tests::Result"] == "PASSED"
+    assert kv["Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon::Result"] == "PASSED"
+    assert kv["Iteration Count Information::Result"] == "PASSED"
+    assert int(kv["Iteration Count Information::Optimized CG iterations per set"]) <= 52
+    assert kv["Reproducibility Information::Result"] == "PASSED"
+    assert float(kv["Reproducibility Information::Scaled residual variance"]) == 0.0
+    assert "Reference version of" not in text   # every hook is the device one
