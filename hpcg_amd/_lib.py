@@ -44,6 +44,8 @@ SIGNATURES = {
     "hpcgx_vec_upload": (C.c_int, [_vp, C.c_int, _vp, _dp, C.c_int64]),
     "hpcgx_vec_download": (C.c_int, [_vp, C.c_int, _vp, _dp, C.c_int64]),
     "hpcgx_vec_fill": (C.c_int, [_vp, C.c_int, _vp, C.c_double]),
+    "hpcgx_vec_from_natural": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "hpcgx_vec_to_natural": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "hpcgx_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "hpcgx_symgs": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "hpcgx_restrict": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
@@ -220,6 +222,15 @@ class Context:
         out = np.empty(self.info(level)["n"])
         self._call("hpcgx_vec_download", self.h, level, _ptr(v), out, out.size)
         return out
+
+    def from_natural(self, v, nat, level=None):
+        """device natural-order buffer (torch tensor / pointer) -> device vector v"""
+        level = v.level if level is None else level
+        self._call("hpcgx_vec_from_natural", self.h, level, _ptr(v), _ptr(nat))
+
+    def to_natural(self, v, nat, level=None):
+        level = v.level if level is None else level
+        self._call("hpcgx_vec_to_natural", self.h, level, _ptr(v), _ptr(nat))
 
     def fill(self, v, value, level=None):
         level = v.level if level is None else level

@@ -585,7 +585,7 @@ const char *hpcgx_error_string(const hpcgx_ctx *c) { return c ? c->err.c_str() :
 int hpcgx_set_stream(hpcgx_ctx *c, void *stream) {
   if (!c) return HPCGX_EINVAL;
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  c->stream = (hipStream_t)stream;
   return HPCGX_OK;
 }
 
@@ -781,6 +781,26 @@ int hpcgx_vec_download(hpcgx_ctx *c, int level, const double *dvec, double *host
   HPCGX_CHECK_LAUNCH(c);
   HPCGX_HIP(c, hipMemcpyAsync(host, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  return HPCGX_OK;
+}
+
+int hpcgx_vec_from_natural(hpcgx_ctx *c, int level, double *dvec, const double *dnat) {
+  if (!c || !dvec || !dnat || level < 0 || level >= (int)c->levels.size() || dvec == dnat)
+    return set_err(c, HPCGX_EINVAL, "vec_from_natural: bad argument");
+  Level &L = c->levels[level];
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  k_gather_natural<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(dvec, dnat, L.iperm, L.lay.npad);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+int hpcgx_vec_to_natural(hpcgx_ctx *c, int level, const double *dvec, double *dnat) {
+  if (!c || !dvec || !dnat || level < 0 || level >= (int)c->levels.size() || dvec == dnat)
+    return set_err(c, HPCGX_EINVAL, "vec_to_natural: bad argument");
+  Level &L = c->levels[level];
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  k_scatter_natural<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(dnat, dvec, L.perm, L.lay.n);
+  HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
 

@@ -59,7 +59,8 @@ int hpcgx_create(int device, hpcgx_ctx **out);
  * SparseMatrix.hpp:151-183, for the device copies). */
 int hpcgx_destroy(hpcgx_ctx *ctx);
 const char *hpcgx_error_string(const hpcgx_ctx *ctx);
-/* Orders all later work on `stream` (a hipStream_t; NULL = the context's own). */
+/* Orders all later work on `stream` (a hipStream_t; NULL is the null stream).
+ * A new context works on a non-blocking stream of its own. */
 int hpcgx_set_stream(hpcgx_ctx *ctx, void *stream);
 int hpcgx_sync(hpcgx_ctx *ctx);
 
@@ -116,6 +117,11 @@ int hpcgx_vec_free(hpcgx_ctx *ctx, double *dvec);
 int hpcgx_vec_upload(hpcgx_ctx *ctx, int level, double *dvec, const double *host, int64_t n);
 /* device (colour order) -> host (natural order, n entries) */
 int hpcgx_vec_download(hpcgx_ctx *ctx, int level, const double *dvec, double *host, int64_t n);
+/* Device-to-device order conversion: dnat (natural order, n entries, device)
+ * -> dvec (colour order, padded), and back.  For callers that keep their own
+ * device buffers (e.g. torch tensors). */
+int hpcgx_vec_from_natural(hpcgx_ctx *ctx, int level, double *dvec, const double *dnat);
+int hpcgx_vec_to_natural(hpcgx_ctx *ctx, int level, const double *dvec, double *dnat);
 /* ZeroVector (Vector.hpp:55-60) and friends: every true entry := value. */
 int hpcgx_vec_fill(hpcgx_ctx *ctx, int level, double *dvec, double value);
 
