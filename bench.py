@@ -12,6 +12,13 @@ timed with HIP events on the stream it runs on), time to relative residual
 timed on this host on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--grid N] [--gpus 1]
+
+The other configurations of the workload print their own JSON line:
+    --config 2   512^3: FP64 vs mixed-precision time to 1e-9, kernel HBM rates
+    --config 3   384^3 variable-coefficient operator, FP16 MG: forward error
+                 vs an FP64 solve at 1e-12
+    --config 4   128^3 adjoint through torch.autograd: backward/forward time,
+                 finite differences on 64 kappa entries, fp64 CPU gradient
 """
 import argparse
 import json
@@ -27,6 +34,9 @@ HBM_PEAK_GBS = 8000.0
 # the reference's 50-iteration scaled residual (CG_ref, natural-order
 # Gauss-Seidel), measured from the reference build at these grids
 REF_REDUCTION = {16: 3.66234e-41, 192: 5.31511e-05}
+# HBM bytes per launch of the fine-level colour step, from the calibrated
+# FETCH_SIZE / WRITE_SIZE runs in profiles/k_gs_fine_192_fp64_pmc_traffic.md
+TRAFFIC_PER_LAUNCH = {192: 359538215}
 
 
 def log(*a):
@@ -72,11 +82,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-1e-9 solves")
     ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4))
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="first copy 1 GiB with a plain copy kernel (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == 2:
+        return config_mixed_512(args)
+    if args.config == 3:
+        return config_kappa_fp16(args)
+    if args.config == 4:
+        return config_adjoint(args)
 
     import numpy as np
     import torch
@@ -86,6 +106,14 @@ def main():
 
     g, it = args.grid, args.iters
     torch.cuda.init()
+    if args.pmc_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] calibration: 3 copies of 1 GiB (1048576 KiB read + 1048576 KiB written each)")
     stream = torch.cuda.Stream()  # default priority; every kernel below runs on it
     ctx = hpcg_amd.Context(0)
     ctx.set_stream(stream.cuda_stream)
@@ -145,7 +173,7 @@ def main():
     achieved = gs_bytes / nlaunch / t_launch / 1e9
     roof = {"kernel": "k_gs<double,double,double,-1,true> (fine-level Gauss-Seidel colour step)",
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_PER_LAUNCH.get(g),
             "avg_launch_us": round(t_launch * 1e6, 2), "bytes_per_launch": int(gs_bytes / nlaunch)}
     log(f"[bench] fine GS step: {t_launch * 1e6:.1f} us/launch, {achieved:.0f} GB/s")
 
@@ -198,6 +226,230 @@ def main():
         "cpu_baseline": cpu,
         "time_to_rel_res_1e-9": tol_res,
     }
+    print(json.dumps(out), flush=True)
+
+
+def solve_timed(ctx, db, dx, variant, tol, reps=1, max_iter=5000):
+    best, out = None, None
+    for _ in range(reps):
+        ctx.fill(dx, 0.0)
+        ctx.sync()
+        t = time.perf_counter()
+        out = ctx.cg(db, dx, max_iter=max_iter, tol=tol, variant=variant)
+        ctx.sync()
+        el = time.perf_counter() - t
+        best = el if best is None else min(best, el)
+    out["seconds"] = best
+    return out
+
+
+def true_relres(ctx, db, dx, bnorm):
+    r = ctx.vec(0)
+    ctx.spmv(0, dx, r)
+    rr = ctx.waxpby_dot(1.0, db, -1.0, r, r) ** 0.5
+    r.free()
+    return rr / bnorm
+
+
+def config_mixed_512(args):
+    """512^3 HPCG problem, FP64 MG-PCG vs FP64-outer + FP32 (and
+    scaled FP16) MG to relative residual 1e-9, verified in FP64."""
+    import numpy as np
+    import torch
+
+    import hpcg_amd
+    from hpcg_amd.models import algorithmic_bytes_per_iteration, hpcg_flops, hpcg_rhs, level_shapes
+    g = args.grid if args.grid != 192 else 512
+    stream = torch.cuda.Stream()
+    ctx = hpcg_amd.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    t = time.perf_counter()
+    ctx.generate(g, g, g, levels=4)
+    ctx.prepare(hpcg_amd.HPCGX_PREC_FP32)
+    ctx.prepare(hpcg_amd.HPCGX_PREC_FP16)
+    ctx.sync()
+    setup = time.perf_counter() - t
+    b = hpcg_rhs(g, g, g)
+    bn = float(np.linalg.norm(b))
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    log(f"[bench c2] {g}^3 setup {setup:.1f} s, device {ctx.memory_bytes() / 1e9:.1f} GB")
+    res = {}
+    shapes = level_shapes(g, g, g)
+    for name, variant, vmg, smg in (("fp64", hpcg_amd.HPCGX_CG_FP64, 8, 8),
+                                    ("mixed_fp32", hpcg_amd.HPCGX_CG_MIXED_FP32, 4, 4),
+                                    ("mixed_fp16", hpcg_amd.HPCGX_CG_MIXED_FP16, 2, 4)):
+        o = solve_timed(ctx, db, dx, variant, 1e-9)
+        rel = true_relres(ctx, db, dx, bn)
+        per_it = o["seconds"] / max(1, o["niters"])
+        by = algorithmic_bytes_per_iteration(g, g, g, v_mg=vmg, s_mg=smg)
+        res[name] = {"seconds": round(o["seconds"], 3), "iterations": o["niters"],
+                     "true_rel_residual": float(f"{rel:.4e}"), "ms_per_iteration": round(per_it * 1e3, 3),
+                     "algorithmic_tb_s": round(by / per_it / 1e12, 3)}
+        log(f"[bench c2] {name}: {o['niters']} it, {o['seconds']:.2f} s, rel res {rel:.3e}, "
+            f"{per_it * 1e3:.2f} ms/it, {by / per_it / 1e12:.2f} TB/s")
+    # the fp64 sets at this size, rated like the headline
+    ctx.fill(dx, 0.0)
+    ctx.sync()
+    t = time.perf_counter()
+    ctx.cg(db, dx, max_iter=50, tol=0.0)
+    ctx.sync()
+    el = time.perf_counter() - t
+    gfl = hpcg_flops(shapes, 50, sets=1)["total"] / el / 1e9
+    dr = ctx.vec(0, np.random.default_rng(0).random(shapes[0][0]))
+    ctx.symgs(0, dr, dx)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(3):
+        ctx.symgs(0, dr, dx)
+    ev1.record(stream)
+    ev1.synchronize()
+    gs_bytes, nl = fine_gs_bytes(g)
+    tl = ev0.elapsed_time(ev1) * 1e-3 / (3 * nl)
+    out = {"metric": METRIC, "config": {"workload": f"configs[2]: HPCG {g}^3, FP64 MG-PCG vs FP64-outer "
+                                                    f"flexible PCG with FP32 / scaled-FP16 MG, to rel-res 1e-9",
+                                        "grid": [g, g, g], "levels": 4},
+           "value": res["fp64"]["seconds"] / res["mixed_fp32"]["seconds"], "unit": "x (time to 1e-9, FP64 / mixed FP32)",
+           "time_to_rel_res_1e-9": res, "fp64_gflops_50_iterations": round(gfl, 1),
+           "speedup_fp16_vs_fp64": round(res["fp64"]["seconds"] / res["mixed_fp16"]["seconds"], 3),
+           "fine_gs_step": {"avg_launch_us": round(tl * 1e6, 1), "achieved_gb_s": round(gs_bytes / nl / tl / 1e9, 1),
+                            "frac_of_8tb_s": round(gs_bytes / nl / tl / 8e12, 4)},
+           "device_gb": round(ctx.memory_bytes() / 1e9, 2), "setup_s": round(setup, 2)}
+    out["value"] = round(out["value"], 3)
+    print(json.dumps(out), flush=True)
+
+
+def kappa_field(g, seed=2026):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    kappa = np.exp(0.5 * rng.standard_normal(g ** 3))
+    b = rng.uniform(-1.0, 1.0, g ** 3)
+    return kappa, b
+
+
+def config_kappa_fp16(args):
+    """384^3 variable-coefficient operator (log kappa ~ N(0, 0.5^2),
+    b ~ U(-1,1), seeded); MG values stored as scaled FP16 with FP32
+    accumulation, FP64 outer CG.  Forward error against an FP64 solve at 1e-12."""
+    import numpy as np
+    import torch
+
+    import hpcg_amd
+    g = args.grid if args.grid != 192 else 384
+    kappa, b = kappa_field(g)
+    stream = torch.cuda.Stream()
+    ctx = hpcg_amd.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    t = time.perf_counter()
+    ctx.generate(g, g, g, levels=4, kappa=kappa)
+    ctx.prepare(hpcg_amd.HPCGX_PREC_FP16)
+    ctx.prepare(hpcg_amd.HPCGX_PREC_FP32)
+    ctx.sync()
+    setup = time.perf_counter() - t
+    bn = float(np.linalg.norm(b))
+    db, dx, dref = ctx.vec(0, b), ctx.vec(0), ctx.vec(0)
+    ref = solve_timed(ctx, db, dref, hpcg_amd.HPCGX_CG_FP64, 1e-12)
+    xref = ctx.download(dref)
+    rows = {"fp64_1e-12": {"seconds": round(ref["seconds"], 3), "iterations": ref["niters"],
+                           "true_rel_residual": float(f"{true_relres(ctx, db, dref, bn):.4e}")}}
+    log(f"[bench c3] fp64 reference: {ref['niters']} it {ref['seconds']:.2f} s")
+    for name, variant in (("mixed_fp16", hpcg_amd.HPCGX_CG_MIXED_FP16), ("mixed_fp32", hpcg_amd.HPCGX_CG_MIXED_FP32),
+                          ("fp64", hpcg_amd.HPCGX_CG_FP64)):
+        for tol in (1e-9, 1e-12):
+            o = solve_timed(ctx, db, dx, variant, tol)
+            x = ctx.download(dx)
+            fe = float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
+            rel = true_relres(ctx, db, dx, bn)
+            rows[f"{name}_{tol:g}"] = {"seconds": round(o["seconds"], 3), "iterations": o["niters"],
+                                       "true_rel_residual": float(f"{rel:.4e}"), "forward_error": float(f"{fe:.4e}")}
+            log(f"[bench c3] {name} tol {tol:g}: {o['niters']} it {o['seconds']:.2f} s rel {rel:.2e} fwd err {fe:.2e}")
+    out = {"metric": METRIC, "config": {"workload": f"configs[3]: variable-coefficient 27-point operator {g}^3, "
+                                                    f"log kappa ~ N(0, 0.25), b ~ U(-1,1); MG in scaled FP16 values, "
+                                                    f"FP32 arithmetic, FP64 outer PCG", "grid": [g, g, g]},
+           "value": rows["mixed_fp16_1e-09"]["forward_error"], "unit": "forward error at rel-res 1e-9 (FP16 MG)",
+           "higher_is_better": False, "solves": rows, "setup_s": round(setup, 2),
+           "speedup_fp16_vs_fp64_1e-9": round(rows["fp64_1e-09"]["seconds"] / rows["mixed_fp16_1e-09"]["seconds"], 3)}
+    print(json.dumps(out), flush=True)
+
+
+def config_adjoint(args):
+    """configs[4]: adjoint through the solve at 128^3 (kappa, b as configs[3]);
+    L = 1/2 |x - xt|^2 with xt a seeded N(0,1) field smoothed by 3 SYMGS
+    sweeps.  Forward and adjoint solves: mixed FP32-MG / FP64-outer to 1e-10."""
+    import numpy as np
+    import torch
+
+    import hpcg_amd
+    from hpcg_amd.adjoint import KappaOperator, KappaSolve, kappa_solve
+    g = args.grid if args.grid != 192 else 128
+    kappa, b = kappa_field(g)
+    rng = np.random.default_rng(77)
+    # target: seeded N(0,1) field, 3 symmetric Gauss-Seidel sweeps with r = 0 on the operator
+    op = KappaOperator(kappa, (g, g, g), variant="fp64")
+    xt = torch.tensor(rng.standard_normal(g ** 3), device="cuda")
+    xd, zr = op._dev(None), torch.zeros(op.npad, dtype=torch.float64, device="cuda")
+    op.ctx.from_natural(xd, xt, level=0)
+    for _ in range(3):
+        op.ctx.symgs(0, zr, xd)
+    op.ctx.to_natural(xd, xt, level=0)
+    op.ctx.sync()
+    for rep in range(2):  # the first pass warms torch's kernels up; the second is timed
+        k = torch.tensor(kappa, device="cuda", requires_grad=True)
+        bt = torch.tensor(b, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x = kappa_solve(k, bt, (g, g, g), 1e-10, "mixed_fp32")
+        loss = 0.5 * ((x - xt) ** 2).sum()
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t
+        t = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t
+    tim = dict(KappaSolve.last_timing)
+    gk = k.grad.detach()
+    log(f"[bench c4] forward {t_fwd:.3f} s backward {t_bwd:.3f} s timing {tim}")
+    # finite differences on 64 kappa entries, fp64 solves to 1e-13
+    bd = torch.tensor(b, device="cuda")
+    idx = rng.choice(g ** 3, 64, replace=False)
+    h = 1e-6
+    fd = []
+    for j in idx:
+        xs = []
+        for sgn in (1.0, -1.0):
+            kp = kappa.copy()
+            kp[j] += sgn * h
+            o = KappaOperator(kp, (g, g, g), variant="fp64")
+            xs.append(o.solve(bd, tol=1e-13))
+            o.close()
+        fd.append(0.5 * torch.dot(xs[0] - xs[1], xs[0] + xs[1] - 2 * xt).item() / (2 * h))
+    fd = np.array(fd)
+    ga = gk[torch.tensor(idx, device="cuda")].cpu().numpy()
+    fd_rel = float(np.max(np.abs(fd - ga)) / np.max(np.abs(ga)))
+    log(f"[bench c4] FD max rel err (vs max |g|) {fd_rel:.2e}")
+    # fp64 CPU reference gradient (the oracle, fp64 CG to 1e-13)
+    cpu_rel = None
+    try:
+        from oracle import pyoracle as po
+        P = po.Problem(g, g, g, kappa=kappa)
+        xtn = xt.cpu().numpy()
+        xo = P.cg(b, max_iter=2000, tol=1e-13, mode=2)["x"]
+        lo = P.cg(xo - xtn, max_iter=2000, tol=1e-13, mode=2)["x"]
+        go = P.kappa_grad(lo, xo)
+        cpu_rel = float(np.linalg.norm(gk.cpu().numpy() - go) / np.linalg.norm(go))
+        b_rel = float(np.linalg.norm(bt.grad.cpu().numpy() - lo) / np.linalg.norm(lo))
+        log(f"[bench c4] vs fp64 CPU gradient: dL/dkappa rel {cpu_rel:.2e}, dL/db rel {b_rel:.2e}")
+    except Exception as e:
+        log(f"[bench c4] CPU gradient failed: {e}")
+        b_rel = None
+    out = {"metric": METRIC, "config": {"workload": f"configs[4]: adjoint of the {g}^3 variable-coefficient solve "
+                                                    f"through torch.autograd.Function, mixed FP32-MG / FP64-outer "
+                                                    f"PCG to 1e-10", "grid": [g, g, g]},
+           "value": round(t_bwd / t_fwd, 3), "unit": "backward / forward time (autograd calls, incl. setup)",
+           "higher_is_better": False,
+           "solve_ratio_adjoint_over_forward": round(tim["adjoint_solve_s"] / tim["forward_solve_s"], 3),
+           "forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4), "timing": tim,
+           "fd_64_entries_max_rel_err": fd_rel, "fp64_cpu_gradient_rel_err": cpu_rel,
+           "fp64_cpu_db_rel_err": b_rel}
     print(json.dumps(out), flush=True)
 
 

@@ -32,6 +32,8 @@ SIGNATURES = {
     "hpcgx_set_colour_order": (C.c_int, [_vp, _ip]),
     "hpcgx_level_upload": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "hpcgx_level_upload_packed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _u8p, _ip, _dp, _vp]),
+    "hpcgx_level_upload_rows": (C.c_int, [_vp, C.c_int, C.c_int64, _u8p, _ip, _dp, _vp, C.c_int64]),
+    "hpcgx_level_colouring": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
     "hpcgx_problem_generate": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "hpcgx_level_info": (C.c_int, [_vp, C.c_int, _i64p]),
     "hpcgx_level_set_diagonal": (C.c_int, [_vp, C.c_int, _dp]),
@@ -167,6 +169,24 @@ class Context:
         fp = None if f2c_arr is None else f2c_arr.ctypes.data_as(C.c_void_p)
         self._call("hpcgx_level_upload_packed", self.h, level, dims[0], dims[1], dims[2],
                    nnz_row, cols, vals, fp)
+
+    def upload_rows(self, level, nnz_row, cols, vals, f2c=None, n_coarse=0):
+        """General rows without a grid (hpcgx_level_upload_rows)."""
+        nnz_row = np.ascontiguousarray(nnz_row, np.uint8)
+        cols = np.ascontiguousarray(cols, np.int32).reshape(-1)
+        vals = np.ascontiguousarray(vals, np.float64).reshape(-1)
+        f2c_arr = None if f2c is None else np.ascontiguousarray(f2c, np.int32)
+        fp = None if f2c_arr is None else f2c_arr.ctypes.data_as(C.c_void_p)
+        self._call("hpcgx_level_upload_rows", self.h, level, len(nnz_row), nnz_row, cols, vals, fp,
+                   0 if f2c_arr is None else (n_coarse or len(f2c_arr)))
+
+    def colouring(self, level=0):
+        """(colour of each natural row, number of colours, general layout?)"""
+        nc, gen = C.c_int(0), C.c_int(0)
+        col = np.zeros(self.info(level)["n"], np.int32)
+        self._call("hpcgx_level_colouring", self.h, level, C.byref(nc), C.byref(gen),
+                   col.ctypes.data_as(C.c_void_p))
+        return col, nc.value, bool(gen.value)
 
     def info(self, level=0):
         out = np.zeros(8, np.int64)

@@ -48,8 +48,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const int *__restrict__ 
                                                         const double *__restrict__ vals,
                                                         const double *__restrict__ p,
                                                         double *__restrict__ Ap, int64_t nrows,
-                                                        double *partials) {
-  const int64_t row = tile_id() * kBlock + threadIdx.x;
+                                                        int64_t block_rows, double *partials) {
+  const int64_t row = map_row(block_rows, tile_id());
   double acc = 0.0;
   if (row < nrows) {
     const size_t b = ell_base(row);
@@ -162,8 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_copy_z(int64_t n, const float *__res
 __global__ __launch_bounds__(kBlock) void k_kappa_grad(const int *__restrict__ cols, const int *__restrict__ iperm,
                                                        const double *__restrict__ lam,
                                                        const double *__restrict__ x, double *__restrict__ g,
-                                                       int64_t npad) {
-  const int64_t p = tile_id() * kBlock + threadIdx.x;
+                                                       int64_t npad, int64_t block_rows) {
+  const int64_t p = map_row(block_rows, tile_id());
   if (p >= npad) return;
   if (iperm[p] < 0) {
     g[p] = 0.0;
@@ -255,7 +255,7 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
     tm.mark(2);
     k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, k == 1);
     tm.mark(3);
-    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials);
+    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, L0.lay.interleave, c->partials);
     finish(c, 1, T, F_PAP);
     tm.mark(4);
     k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->scal);
@@ -327,7 +327,7 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
       k_cg_p<float><<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, c->scal, 0);
     restart = false;
     tm.mark(3);
-    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, c->partials);
+    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, L0.lay.interleave, c->partials);
     finish(c, 1, T, F_PAP);
     tm.mark(4);
     k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal);
@@ -383,7 +383,8 @@ int hpcgx_kappa_grad(hpcgx_ctx *c, const double *lam, const double *x, double *g
   int rc = ensure_ready(c);
   if (rc) return rc;
   const Level &L = c->levels[0];
-  k_kappa_grad<<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.iperm, lam, x, g, L.lay.npad);
+  k_kappa_grad<<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.iperm, lam, x, g, L.lay.npad,
+                                                            L.lay.interleave);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }

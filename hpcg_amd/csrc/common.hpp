@@ -17,7 +17,8 @@ constexpr int kSlots = 27;     // one slot per stencil offset (dx,dy,dz) in {-1,
 constexpr int kSlice = 64;     // rows per ELL slice = one wavefront
 constexpr int kDiagSlot = 13;  // slot of offset (0,0,0)
 constexpr int kBlock = 256;    // threads per block = 4 slices
-constexpr int kColours = 8;
+constexpr int kColours = 8;     // parity classes of the stencil layout
+constexpr int kMaxColours = 64; // colours of a general (unstructured) level
 constexpr int kMaxLevels = 8;
 
 // Device scalars of the CG recurrences (Ctx::scal).
@@ -45,19 +46,28 @@ enum Scalar : int {
 // reference's natural-order Gauss-Seidel at 32^3..96^3.
 constexpr int kDefaultOrder[kColours] = {7, 3, 4, 5, 2, 6, 1, 0};
 
-// Row layout of one level.  Rows are grouped in 8 blocks, block k holding the
-// rows of parity class cls[k].  Inside a block rows keep natural order of
-// (x>>1, y>>1, z>>1).  Each block is padded to a multiple of 64 rows; padded
-// rows hold the identity row (diagonal 1, other slots 0) and zero vectors.
+// Row layout of one level.  Stencil layout: rows are grouped in 8 blocks,
+// block k holding the rows of parity class cls[k]; inside a block rows keep
+// natural order of (x>>1, y>>1, z>>1), and slot s of a row is the stencil
+// offset s.  General layout (rows that are not a 27-point stencil of their
+// grid): ncol blocks from a greedy colouring, rows ascending inside a block,
+// and slot j of a row is the row's j-th entry.  Each block is padded to a
+// multiple of 64 rows; padded rows hold the identity row (diagonal 1 in the
+// diagonal slot, other slots 0) and zero vectors.
 struct Layout {
   int nx = 0, ny = 0, nz = 0;
   int64_t n = 0, npad = 0;
-  int64_t start[kColours + 1] = {};  // padded block starts
-  int64_t count[kColours] = {};      // true rows per block
-  int nb[kColours][3] = {};          // block extents in x, y, z
-  int cls[kColours] = {};            // parity class of block k
-  int pos[kColours] = {};            // block of parity class c
-  int inj = 0;                       // block of class 0 (the injected rows)
+  int ncol = kColours;                   // colour blocks
+  bool general = false;                  // greedy colouring of arbitrary rows
+  int64_t start[kMaxColours + 1] = {};   // padded block starts
+  int64_t count[kMaxColours] = {};       // true rows per block
+  // stencil layout only
+  int nb[kColours][3] = {};              // block extents in x, y, z
+  int cls[kColours] = {};                // parity class of block k
+  int pos[kColours] = {};                // block of parity class c
+  int inj = 0;                           // block of class 0 (the injected rows)
+  int64_t interleave = 0;                // padded rows per block if all 8 are equal and
+                                         // a multiple of 256 (map_row), else 0
 };
 
 Layout make_layout(int nx, int ny, int nz, const int *order);
@@ -88,16 +98,30 @@ struct Level {
   std::vector<int> perm_h;   // natural -> colour order [n]
   std::vector<int> iperm_h;  // colour order -> natural, -1 on padding [npad]
   std::vector<int> f2c_h;    // natural f2c of this level (uploaded levels), [n_coarse]
+  std::vector<uint8_t> rowlen_h;  // general levels: entries per natural row
   // device matrix, slot-major inside 64-row slices: (slice*27+slot)*64+lane
   int *cols = nullptr;
   double *vals = nullptr;
   float *vals32 = nullptr;
   __half *vals16 = nullptr;
+  // compact column indices of the reduced-precision copies: column of
+  // (row, slot) = cbase[slice*27+slot] + lane + cdelta[ell], or cols[] when
+  // cwide[slice] is set (deltas out of int8 range)
+  int *cbase = nullptr;
+  int8_t *cdelta = nullptr;
+  uint8_t *cwide = nullptr;
   float *dinv = nullptr;   // fp16 copy: 1/sqrt(a_ii), colour order
   float *sigma = nullptr;  // fp16 copy: sqrt(d_f/d_c) per injected row
   int *perm = nullptr;
   int *iperm = nullptr;
-  int *inj = nullptr;      // injected row t of this level -> row of level+1
+  // Transfer to level+1, over ninj injected rows t: fine row start[inj] + t
+  // (stencil layout: the injected rows are one colour block) or frow[t]
+  // (general levels); coarse row inj[t].
+  int *inj = nullptr;
+  int *frow = nullptr;
+  int64_t ninj = 0;
+  // general levels: slot of the diagonal in each row (stencil: kDiagSlot)
+  uint8_t *dslot = nullptr;
   // work vectors used when this level is the coarse level of a V-cycle
   double *r = nullptr, *x = nullptr;
   float *r32 = nullptr, *x32 = nullptr;
@@ -120,6 +144,7 @@ struct Ctx {
   // solver buffers (level-0 length)
   double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
   float *cg_r32 = nullptr, *cg_z32 = nullptr;
+  int64_t cg_len = 0, cg32_len = 0;  // elements the solver buffers hold
   double *stage = nullptr;     // natural-order staging for upload/download
   int64_t stage_len = 0;
   std::vector<void *> user_vecs;

@@ -28,6 +28,18 @@ __device__ __forceinline__ int64_t tile_id() {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Row of thread threadIdx.x in the tile-th 256-row tile of a whole-level
+// kernel.  When the 8 colour blocks have equal padded sizes that are a
+// multiple of 256 (even grids), tiles interleave the blocks (tile t -> block
+// t % 8, tile t / 8 inside it): concurrently running tiles then cover all 8
+// classes of one region of the grid, so a gathered vector entry is fetched
+// once and reused by its neighbours of every colour, instead of once per
+// colour block.  Otherwise rows run in storage order.
+__device__ __forceinline__ int64_t map_row(int64_t block_rows, int64_t tile) {
+  if (block_rows == 0) return tile * kBlock + threadIdx.x;
+  return (tile & 7) * block_rows + (tile >> 3) * kBlock + threadIdx.x;
+}
+
 // Matrix loads.  NT (non-temporal): the fine level's matrix is streamed once
 // per pass and is far larger than any cache, so it is kept out of the way of
 // the vectors; coarse-level matrices fit in the 256 MB Infinity Cache and
@@ -54,6 +66,36 @@ __device__ __forceinline__ ACC ld_val(const __half *p) {
   if constexpr (NT) bits = __builtin_nontemporal_load(q);
   else bits = *q;
   return (ACC)__half2float(__ushort_as_half(bits));
+}
+
+// Column index of (row, slot): explicit 32-bit index, or for the compact
+// copies base + lane + int8 delta with a per-slice fallback.
+struct ColIdx {
+  const int *cols;
+  const int *cbase;     // null: explicit indices only
+  const int8_t *cdelta;
+  const uint8_t *cwide;
+};
+
+template <bool CI, bool NT>
+__device__ __forceinline__ int load_col(const ColIdx &ci, int64_t row, size_t b, int s, bool wide) {
+  if constexpr (!CI) {
+    return ld_col<NT>(ci.cols + b + s * kSlice);
+  } else {
+    if (wide) return ld_col<NT>(ci.cols + b + s * kSlice);
+    const int64_t slice = row >> 6;
+    const int base = ci.cbase[slice * kSlots + s];
+    int8_t d;
+    if constexpr (NT) d = __builtin_nontemporal_load(ci.cdelta + b + s * kSlice);
+    else d = ci.cdelta[b + s * kSlice];
+    return base + (int)(row & 63) + (int)d;
+  }
+}
+
+template <bool CI>
+__device__ __forceinline__ bool slice_wide(const ColIdx &ci, int64_t row) {
+  if constexpr (!CI) return false;
+  else return ci.cwide[row >> 6] != 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -171,10 +213,9 @@ template <class VT, class XT, class ACC>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ cols,
                                                  const VT *__restrict__ vals,
                                                  const XT *__restrict__ x, XT *__restrict__ y,
-                                                 int64_t row0, int64_t nrows) {
-  const int64_t t = tile_id() * kBlock + threadIdx.x;
-  if (t >= nrows) return;
-  const int64_t row = row0 + t;
+                                                 int64_t nrows, int64_t block_rows) {
+  const int64_t row = map_row(block_rows, tile_id());
+  if (row >= nrows) return;
   const size_t b = ell_base(row);
   ACC sum = 0;
 #pragma unroll
@@ -207,22 +248,27 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
 // compile-time constant, so the remaining loads issue back to back.
 // FINE: the finest level (streamed matrix loads; its own instantiation, so
 // profiles show the dominant launches separately from the coarse levels').
-template <class VT, class XT, class ACC, int ZK, bool FINE>
-__global__ __launch_bounds__(kBlock) void k_gs(const int *__restrict__ cols,
-                                               const VT *__restrict__ vals,
+// GEN: general level, the diagonal's slot is dslot[row] (stencil: slot 13).
+template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false>
+__global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
                                                const XT *__restrict__ r, XT *__restrict__ x,
-                                               int64_t row0, int64_t nrows) {
+                                               int64_t row0, int64_t nrows,
+                                               const uint8_t *__restrict__ dslot) {
+  static_assert(!GEN || ZK < 0, "general levels sweep full rows");
   constexpr uint32_t mask = ZK >= 0 ? swept_before_mask(ZK) : 0x7ffffffu;
   const int64_t t = tile_id() * kBlock + threadIdx.x;
   if (t >= nrows) return;
   const int64_t row = row0 + t;
   const size_t b = ell_base(row);
-  const ACC d = ld_val<ACC, FINE>(vals + b + kDiagSlot * kSlice);
+  const bool wide = slice_wide<CI>(ci, row);
+  int ds = kDiagSlot;
+  if constexpr (GEN) ds = dslot[row];
+  const ACC d = ld_val<ACC, FINE>(vals + b + ds * kSlice);
   ACC sum = (ACC)r[row];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) {
     if (!((mask >> s) & 1u)) continue;
-    const int c = ld_col<FINE>(cols + b + s * kSlice);
+    const int c = load_col<CI, FINE>(ci, row, b, s, wide);
     const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
     sum -= v * (ACC)x[c];
   }
@@ -232,24 +278,24 @@ __global__ __launch_bounds__(kBlock) void k_gs(const int *__restrict__ cols,
 
 // Residual at the injected rows and restriction, fused (ComputeMG_ref.cpp:49-51 +
 // ComputeRestriction_ref.cpp:51): rc[inj[t]] = r_f - (A x)_f for the t-th
-// injected row f = row0 + t.  (A x)_f is summed from 0 like SpMV, then
-// subtracted.  SCALE: fp16 V-cycle, times sigma[t].
-template <class VT, class XT, class ACC, bool SCALE>
-__global__ __launch_bounds__(kBlock) void k_restrict(const int *__restrict__ cols,
-                                                     const VT *__restrict__ vals,
+// injected row f = row0 + t (frow[t] when frow is given).  (A x)_f is summed
+// from 0 like SpMV, then subtracted.  SCALE: fp16 V-cycle, times sigma[t].
+template <class VT, class XT, class ACC, bool SCALE, bool CI = false>
+__global__ __launch_bounds__(kBlock) void k_restrict(const ColIdx ci, const VT *__restrict__ vals,
                                                      const XT *__restrict__ r,
                                                      const XT *__restrict__ x,
                                                      XT *__restrict__ rc, const int *__restrict__ inj,
                                                      const float *__restrict__ sigma, int64_t row0,
-                                                     int64_t count) {
+                                                     int64_t count, const int *__restrict__ frow) {
   const int64_t t = tile_id() * kBlock + threadIdx.x;
   if (t >= count) return;
-  const int64_t row = row0 + t;
+  const int64_t row = frow ? (int64_t)frow[t] : row0 + t;
   const size_t b = ell_base(row);
+  const bool wide = slice_wide<CI>(ci, row);
   ACC ax = 0;
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) {
-    const int c = ld_col(cols + b + s * kSlice);
+    const int c = load_col<CI, true>(ci, row, b, s, wide);
     const ACC v = ld_val<ACC>(vals + b + s * kSlice);
     ax += v * (ACC)x[c];
   }
@@ -263,10 +309,10 @@ template <class XT, class ACC, bool SCALE>
 __global__ __launch_bounds__(kBlock) void k_prolong(XT *__restrict__ x, const XT *__restrict__ xc,
                                                     const int *__restrict__ inj,
                                                     const float *__restrict__ sigma, int64_t row0,
-                                                    int64_t count) {
+                                                    int64_t count, const int *__restrict__ frow) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
-  const int64_t row = row0 + t;
+  const int64_t row = frow ? (int64_t)frow[t] : row0 + t;
   ACC v = (ACC)xc[inj[t]];
   if (SCALE) v = v * (ACC)sigma[t];
   x[row] = (XT)((ACC)x[row] + v);

@@ -16,15 +16,34 @@ const float *vals_of<float>(const Level &L) { return L.vals32; }
 template <>
 const __half *vals_of<__half>(const Level &L) { return L.vals16; }
 
-// One symmetric Gauss-Seidel step over the 8 colour blocks.  Forward sweep
-// blocks 0..7, backward sweep 6..0: backward block 7 would recompute block 7
-// from exactly the neighbour values the forward step used, so it is elided.
-// zero_start: x is known to be zero on entry (the V-cycle's first sweep), so
-// x is not read before it is written.
+// Column indices of a level's copy in precision VT: the reduced-precision
+// copies use the compact form when it was built.
+template <class VT>
+static ColIdx col_idx(const Level &L) {
+  if (sizeof(VT) < 8 && L.cbase) return ColIdx{L.cols, L.cbase, L.cdelta, L.cwide};
+  return ColIdx{L.cols, nullptr, nullptr, nullptr};
+}
+
 template <class VT, class XT, class ACC, int ZK, bool FINE>
 static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, int k) {
   const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
-  if (nr > 0) k_gs<VT, XT, ACC, ZK, FINE><<<tiles(nr), kBlock, 0, c->stream>>>(L.cols, v, r, x, r0, nr);
+  if (nr <= 0) return;
+  const ColIdx ci = col_idx<VT>(L);
+  const unsigned g = tiles(nr);
+  hipStream_t st = c->stream;
+  if constexpr (ZK < 0) {
+    if (L.dslot) {
+      if (ci.cbase)
+        k_gs<VT, XT, ACC, -1, FINE, true, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, L.dslot);
+      else
+        k_gs<VT, XT, ACC, -1, FINE, false, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, L.dslot);
+      return;
+    }
+  }
+  if (ci.cbase)
+    k_gs<VT, XT, ACC, ZK, FINE, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr);
+  else
+    k_gs<VT, XT, ACC, ZK, FINE, false><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr);
 }
 
 template <class XT>
@@ -34,11 +53,18 @@ __global__ __launch_bounds__(kBlock) void k_zero(XT *__restrict__ x, int64_t n) 
 }
 
 static bool default_order(const Level &L) {
+  if (L.lay.general) return false;
   for (int k = 0; k < kColours; ++k)
     if (L.lay.cls[k] != kDefaultOrder[k]) return false;
   return true;
 }
 
+// One symmetric Gauss-Seidel step over the colour blocks.  Forward sweep
+// blocks 0..ncol-1, backward sweep ncol-2..0: the backward step of the last
+// block would recompute it from exactly the neighbour values its forward step
+// used, so it is elided.  zero_start: x is known to be zero on entry (the
+// V-cycle's first sweep); in the default stencil order x is then not read
+// before it is written, otherwise it is zero-filled first.
 template <class VT, class XT, class ACC, bool FINE>
 static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start) {
   const VT *v = vals_of<VT>(L);
@@ -56,9 +82,9 @@ static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start) {
     gs_block<VT, XT, ACC, 6, FINE>(c, L, v, r, x, 6);
     gs_block<VT, XT, ACC, 7, FINE>(c, L, v, r, x, 7);
   } else {
-    for (int k = 0; k < kColours; ++k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
+    for (int k = 0; k < L.lay.ncol; ++k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
   }
-  for (int k = kColours - 2; k >= 0; --k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
+  for (int k = L.lay.ncol - 2; k >= 0; --k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
 }
 
 template <class VT, class XT, class ACC>
@@ -86,12 +112,17 @@ static int vcycle(Ctx *c, int l, const XT *r, XT *x) {
     rc_v = reinterpret_cast<XT *>(C.r32);
     xc_v = reinterpret_cast<XT *>(C.x32);
   }
-  const int64_t nc = L.lay.count[L.lay.inj], r0 = L.lay.start[L.lay.inj];
-  k_restrict<VT, XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(L.cols, vals_of<VT>(L), r, x, rc_v,
-                                                                       L.inj, L.sigma, r0, nc);
+  const int64_t nc = L.ninj, r0 = L.lay.start[L.lay.inj];
+  const ColIdx ci = col_idx<VT>(L);
+  if (ci.cbase)
+    k_restrict<VT, XT, ACC, SCALE, true><<<tiles(nc), kBlock, 0, c->stream>>>(ci, vals_of<VT>(L), r, x, rc_v,
+                                                                             L.inj, L.sigma, r0, nc, L.frow);
+  else
+    k_restrict<VT, XT, ACC, SCALE, false><<<tiles(nc), kBlock, 0, c->stream>>>(ci, vals_of<VT>(L), r, x, rc_v,
+                                                                              L.inj, L.sigma, r0, nc, L.frow);
   HPCGX_CHECK_LAUNCH(c);
   if ((rc = vcycle<VT, XT, ACC, SCALE>(c, l + 1, rc_v, xc_v))) return rc;
-  k_prolong<XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc_v, L.inj, L.sigma, r0, nc);
+  k_prolong<XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc_v, L.inj, L.sigma, r0, nc, L.frow);
   HPCGX_CHECK_LAUNCH(c);
   return symgs_launch<VT, XT, ACC>(c, l, r, x, false);
 }
@@ -193,8 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_to_double(const float *__restrict__ 
 
 int spmv_fp64(Ctx *c, int level, const double *x, double *y) {
   const Level &L = c->levels[level];
-  k_spmv<double, double, double><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, x, y, 0,
-                                                                              L.lay.npad);
+  k_spmv<double, double, double><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, x, y,
+                                                                              L.lay.npad, L.lay.interleave);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -236,9 +267,10 @@ int hpcgx_restrict(hpcgx_ctx *c, int level, const double *r, const double *x, do
   if (rc) return rc;
   if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "restrict: level has no coarse level");
   const Level &L = c->levels[level];
-  const int64_t nc = L.lay.count[L.lay.inj];
-  k_restrict<double, double, double, false><<<tiles(nc), kBlock, 0, c->stream>>>(
-      L.cols, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[L.lay.inj], nc);
+  const int64_t nc = L.ninj;
+  k_restrict<double, double, double, false, false><<<tiles(nc), kBlock, 0, c->stream>>>(
+      ColIdx{L.cols, nullptr, nullptr, nullptr}, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[L.lay.inj], nc,
+      L.frow);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -248,9 +280,9 @@ int hpcgx_prolong(hpcgx_ctx *c, int level, const double *xc, double *x) {
   if (rc) return rc;
   if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "prolong: level has no coarse level");
   const Level &L = c->levels[level];
-  const int64_t nc = L.lay.count[L.lay.inj];
+  const int64_t nc = L.ninj;
   k_prolong<double, double, false><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc, L.inj, nullptr,
-                                                                        L.lay.start[L.lay.inj], nc);
+                                                                        L.lay.start[L.lay.inj], nc, L.frow);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }

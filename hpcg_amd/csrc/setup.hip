@@ -4,6 +4,7 @@
 // packing happen once, here; the iteration kernels only read the result.
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -81,6 +82,10 @@ Layout make_layout(int nx, int ny, int nz, const int *order) {
     L.start[k + 1] = L.start[k] + padded;
   }
   L.npad = L.start[kColours];
+  const int64_t b0 = L.start[1] - L.start[0];
+  bool equal = b0 % kBlock == 0;
+  for (int k = 1; k < kColours; ++k) equal = equal && (L.start[k + 1] - L.start[k] == b0);
+  L.interleave = equal ? b0 : 0;
   return L;
 }
 
@@ -137,11 +142,16 @@ static void free_level(Ctx *c, Level &L) {
   dfree(c, L.vals, m * sizeof(double));
   dfree(c, L.vals32, m * sizeof(float));
   dfree(c, L.vals16, m * sizeof(__half));
+  dfree(c, L.cbase, (size_t)(L.lay.npad / kSlice) * kSlots * sizeof(int));
+  dfree(c, L.cdelta, m * sizeof(int8_t));
+  dfree(c, L.cwide, (size_t)(L.lay.npad / kSlice));
   dfree(c, L.dinv, L.lay.npad * sizeof(float));
-  dfree(c, L.sigma, L.lay.count[L.lay.inj] * sizeof(float));
+  dfree(c, L.sigma, L.ninj * sizeof(float));
   dfree(c, L.perm, L.lay.n * sizeof(int));
   dfree(c, L.iperm, L.lay.npad * sizeof(int));
-  dfree(c, L.inj, L.lay.count[L.lay.inj] * sizeof(int));
+  dfree(c, L.inj, L.ninj * sizeof(int));
+  dfree(c, L.frow, L.ninj * sizeof(int));
+  dfree(c, L.dslot, L.lay.npad);
   dfree(c, L.r, L.lay.npad * sizeof(double));
   dfree(c, L.x, L.lay.npad * sizeof(double));
   dfree(c, L.r32, L.lay.npad * sizeof(float));
@@ -265,24 +275,14 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kB
 // ---------------------------------------------------------------------------
 // upload from the reference's structures
 
+// Stencil layout: every row is a subset of the 27-point stencil of its
+// (nx, ny, nz) grid, ascending columns, with a diagonal.  Returns false (and
+// leaves the level for the general layout) when the rows are not.
 template <class RowAcc>
-static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, const int *f2c) {
-  if (level < 0 || level >= kMaxLevels || nx < 1 || ny < 1 || nz < 1)
-    return set_err(c, HPCGX_EINVAL, "level_upload: bad level or grid");
-  if ((int)c->levels.size() < level) return set_err(c, HPCGX_ESTATE, "levels must be uploaded fine to coarse");
-  if ((int)c->levels.size() > level) {
-    for (int l = (int)c->levels.size() - 1; l >= level; --l) free_level(c, c->levels[l]);
-    c->levels.resize(level);
-  }
-  c->ready = false;
-  c->prepared = 0;
-  c->levels.emplace_back();
-  Level &L = c->levels.back();
-  L.lay = make_layout(nx, ny, nz, c->order);
-  L.ldev = to_dev(L.lay);
+static bool pack_stencil(Level &L, RowAcc rows, std::vector<int> &hc, std::vector<double> &hv) {
   const Layout &lay = L.lay;
+  const int nx = lay.nx, ny = lay.ny, nz = lay.nz;
   const int64_t n = lay.n, npad = lay.npad;
-  if (npad >= ((int64_t)1 << 31)) return set_err(c, HPCGX_EINVAL, "level_upload: too many rows for 32-bit columns");
   L.perm_h.assign(n, 0);
   L.iperm_h.assign(npad, -1);
   for (int k = 0; k < kColours; ++k) {
@@ -297,8 +297,8 @@ static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, 
     }
   }
   const size_t m = (size_t)npad * kSlots;
-  std::vector<int> hc(m);
-  std::vector<double> hv(m);
+  hc.assign(m, 0);
+  hv.assign(m, 0.0);
   std::atomic<int> bad(0);
   std::atomic<int64_t> nnz(0);
   parallel_for(npad, [&](int64_t p) {
@@ -308,7 +308,7 @@ static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, 
       hc[b + s * kSlice] = (int)p;
       hv[b + s * kSlice] = (i < 0 && s == kDiagSlot) ? 1.0 : 0.0;
     }
-    if (i < 0) return;
+    if (i < 0 || bad) return;
     const int ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
     const int cnt = rows.nnz(i);
     const int *rc = rows.cols(i);
@@ -331,20 +331,182 @@ static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, 
     if (!diag) { bad = 1; return; }
     nnz += cnt;
   });
-  if (bad) return set_err(c, HPCGX_ENOTSTENCIL, "level_upload: matrix is not a 27-point stencil with ascending columns on the stated grid");
+  if (bad) return false;
   L.nnz = nnz;
+  return true;
+}
+
+// General layout for arbitrary rows (the reference requires every
+// optimisation to work for general sparse matrices, GenerateProblem.cpp:47-49).
+// Colouring: first fit in row order over the symmetrised pattern -- row i
+// takes the smallest colour not held by a j < i with a_ij != 0 or a_ji != 0.
+// On structurally symmetric rows this is the reference's greedy example
+// (OptimizeProblem.cpp:43-79); the transpose term keeps each colour free of
+// couplings when the pattern is not symmetric.  Blocks in colour order, rows
+// ascending inside a block, slot j = the row's j-th entry, so sums run in the
+// reference's order (ComputeSPMV_ref.cpp:67-68, ComputeSYMGS_ref.cpp:74-77).
+template <class RowAcc>
+static int pack_general(Ctx *c, Level &L, RowAcc rows, std::vector<int> &hc, std::vector<double> &hv,
+                        std::vector<uint8_t> &hd) {
+  const int64_t n = L.lay.n;
+  std::atomic<int64_t> bad_row(-1);
+  parallel_for(n, [&](int64_t i) {
+    const int cnt = rows.nnz(i);
+    const int *rc = rows.cols(i);
+    bool ok = cnt >= 1 && cnt <= kSlots, diag = false;
+    for (int j = 0; ok && j < cnt; ++j) {
+      if (rc[j] < 0 || rc[j] >= n) ok = false;
+      else if (rc[j] == i) diag = true;
+    }
+    if (!ok || !diag) bad_row = i;
+  });
+  if (bad_row >= 0)
+    return set_err(c, HPCGX_EINVAL, "level_upload: row " + std::to_string((long long)bad_row) +
+                                         " has no diagonal, a column out of range, or not 1..27 entries");
+  // transpose: rows j < i that hold column i
+  std::vector<int64_t> ts(n + 1, 0);
+  for (int64_t j = 0; j < n; ++j) {
+    const int cnt = rows.nnz(j);
+    const int *rc = rows.cols(j);
+    for (int k = 0; k < cnt; ++k)
+      if (rc[k] > j) ts[rc[k] + 1]++;
+  }
+  for (int64_t i = 0; i < n; ++i) ts[i + 1] += ts[i];
+  std::vector<int> tj(std::max<int64_t>(ts[n], 1));
+  {
+    std::vector<int64_t> fill(ts.begin(), ts.end() - 1);
+    for (int64_t j = 0; j < n; ++j) {
+      const int cnt = rows.nnz(j);
+      const int *rc = rows.cols(j);
+      for (int k = 0; k < cnt; ++k)
+        if (rc[k] > j) tj[fill[rc[k]]++] = (int)j;
+    }
+  }
+  std::vector<int> colour(n);
+  int64_t mark[kMaxColours + 1];
+  for (auto &m : mark) m = -1;
+  int ncol = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int cnt = rows.nnz(i);
+    const int *rc = rows.cols(i);
+    for (int k = 0; k < cnt; ++k)
+      if (rc[k] < i) mark[colour[rc[k]]] = i;
+    for (int64_t t = ts[i]; t < ts[i + 1]; ++t) mark[colour[tj[t]]] = i;
+    int pick = 0;
+    while (mark[pick] == i) ++pick;
+    if (pick >= kMaxColours)
+      return set_err(c, HPCGX_EINVAL, "level_upload: greedy colouring needs more than " +
+                                          std::to_string(kMaxColours) + " colours");
+    colour[i] = pick;
+    ncol = std::max(ncol, pick + 1);
+  }
+  Layout &lay = L.lay;
+  lay.general = true;
+  lay.ncol = ncol;
+  lay.interleave = 0;
+  lay.inj = 0;
+  for (int k = 0; k < kMaxColours; ++k) lay.count[k] = 0;
+  for (int64_t i = 0; i < n; ++i) lay.count[colour[i]]++;
+  lay.start[0] = 0;
+  for (int k = 0; k < ncol; ++k) lay.start[k + 1] = lay.start[k] + (lay.count[k] + kSlice - 1) / kSlice * kSlice;
+  for (int k = ncol + 1; k <= kMaxColours; ++k) lay.start[k] = lay.start[ncol];
+  lay.npad = lay.start[ncol];
+  if (lay.npad >= ((int64_t)1 << 31)) return set_err(c, HPCGX_EINVAL, "level_upload: too many rows for 32-bit columns");
+  const int64_t npad = lay.npad;
+  L.perm_h.assign(n, 0);
+  L.iperm_h.assign(npad, -1);
+  {
+    std::vector<int64_t> fill(lay.start, lay.start + ncol);
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t p = fill[colour[i]]++;
+      L.perm_h[i] = (int)p;
+      L.iperm_h[p] = (int)i;
+    }
+  }
+  const size_t m = (size_t)npad * kSlots;
+  hc.assign(m, 0);
+  hv.assign(m, 0.0);
+  hd.assign(npad, 0);
+  L.rowlen_h.assign(n, 0);
+  std::atomic<int64_t> nnz(0);
+  parallel_for(npad, [&](int64_t p) {
+    const size_t b = (size_t)(p >> 6) * (kSlots * kSlice) + (size_t)(p & 63);
+    const int i = L.iperm_h[p];
+    for (int s = 0; s < kSlots; ++s) {
+      hc[b + s * kSlice] = (int)p;
+      hv[b + s * kSlice] = (i < 0 && s == 0) ? 1.0 : 0.0;
+    }
+    if (i < 0) return;
+    const int cnt = rows.nnz(i);
+    const int *rc = rows.cols(i);
+    const double *rv = rows.vals(i);
+    int ds = -1;
+    for (int j = 0; j < cnt; ++j) {
+      hc[b + j * kSlice] = L.perm_h[rc[j]];
+      hv[b + j * kSlice] = rv[j];
+      if (rc[j] == i && ds < 0) ds = j;
+    }
+    hd[p] = (uint8_t)ds;
+    L.rowlen_h[i] = (uint8_t)cnt;
+    nnz += cnt;
+  });
+  L.nnz = nnz;
+  return HPCGX_OK;
+}
+
+// Uploads one level.  Rows that form a 27-point stencil of the (nx, ny, nz)
+// grid get the parity-class layout; any other rows get the general layout.
+// f2c: natural-order injection of the n_coarse rows of level + 1, or null.
+template <class RowAcc>
+static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, const int *f2c,
+                        int64_t n_coarse, bool try_stencil) {
+  if (level < 0 || level >= kMaxLevels || nx < 1 || ny < 1 || nz < 1)
+    return set_err(c, HPCGX_EINVAL, "level_upload: bad level or grid");
+  if ((int)c->levels.size() < level) return set_err(c, HPCGX_ESTATE, "levels must be uploaded fine to coarse");
+  if ((int)c->levels.size() > level) {
+    for (int l = (int)c->levels.size() - 1; l >= level; --l) free_level(c, c->levels[l]);
+    c->levels.resize(level);
+  }
+  c->ready = false;
+  c->prepared = 0;
+  c->levels.emplace_back();
+  Level &L = c->levels.back();
+  L.lay = make_layout(nx, ny, nz, c->order);
+  L.ldev = to_dev(L.lay);
+  if (L.lay.npad >= ((int64_t)1 << 31) && try_stencil)
+    return set_err(c, HPCGX_EINVAL, "level_upload: too many rows for 32-bit columns");
+  std::vector<int> hc;
+  std::vector<double> hv;
+  std::vector<uint8_t> hd;
   int rc;
+  if (!try_stencil || !pack_stencil(L, rows, hc, hv)) {
+    if ((rc = pack_general(c, L, rows, hc, hv, hd))) return rc;
+  }
+  const Layout &lay = L.lay;
+  const size_t m = (size_t)lay.npad * kSlots;
   if ((rc = alloc_level(c, L))) return rc;
   HPCGX_HIP(c, hipMemcpy(L.cols, hc.data(), m * sizeof(int), hipMemcpyHostToDevice));
   HPCGX_HIP(c, hipMemcpy(L.vals, hv.data(), m * sizeof(double), hipMemcpyHostToDevice));
-  HPCGX_HIP(c, hipMemcpy(L.perm, L.perm_h.data(), n * sizeof(int), hipMemcpyHostToDevice));
-  HPCGX_HIP(c, hipMemcpy(L.iperm, L.iperm_h.data(), npad * sizeof(int), hipMemcpyHostToDevice));
+  HPCGX_HIP(c, hipMemcpy(L.perm, L.perm_h.data(), lay.n * sizeof(int), hipMemcpyHostToDevice));
+  HPCGX_HIP(c, hipMemcpy(L.iperm, L.iperm_h.data(), lay.npad * sizeof(int), hipMemcpyHostToDevice));
+  if (lay.general) {
+    if ((rc = dalloc(c, (void **)&L.dslot, lay.npad, false))) return rc;
+    HPCGX_HIP(c, hipMemcpy(L.dslot, hd.data(), lay.npad, hipMemcpyHostToDevice));
+  }
   if (f2c) {
-    if (nx % 2 || ny % 2 || nz % 2) return set_err(c, HPCGX_EINVAL, "level_upload: f2c given for a grid that cannot be halved");
-    const int64_t nc = (int64_t)(nx / 2) * (ny / 2) * (nz / 2);
-    L.f2c_h.assign(f2c, f2c + nc);
+    if (n_coarse < 1 || n_coarse > lay.n) return set_err(c, HPCGX_EINVAL, "level_upload: bad coarse row count for f2c");
+    L.f2c_h.assign(f2c, f2c + n_coarse);
   }
   return HPCGX_OK;
+}
+
+// Rows of the next level of an nx*ny*nz level: the grid halves in every
+// dimension (GenerateCoarseProblem.cpp:50-53).  Negative: error code.
+static int64_t grid_coarse_rows(Ctx *c, int nx, int ny, int nz, const int *f2c) {
+  if (!f2c) return 0;
+  if (nx < 2 || ny < 2 || nz < 2 || nx % 2 || ny % 2 || nz % 2)
+    return set_err(c, HPCGX_EINVAL, "level_upload: f2c given for a grid that cannot be halved");
+  return (int64_t)(nx / 2) * (ny / 2) * (nz / 2);
 }
 
 struct PackedRows {
@@ -376,25 +538,39 @@ int ensure_ready(Ctx *c) {
     Level &F = c->levels[l], &Cc = c->levels[l + 1];
     const int64_t nc = Cc.lay.n;
     if (F.inj) continue;
-    const int ib = F.lay.inj;
-    if (F.lay.count[ib] != nc || F.lay.nx != 2 * Cc.lay.nx || F.lay.ny != 2 * Cc.lay.ny ||
-        F.lay.nz != 2 * Cc.lay.nz)
-      return set_err(c, HPCGX_ESTATE, "level " + std::to_string(l + 1) + " is not the 2x coarsening of level " + std::to_string(l));
     if ((int64_t)F.f2c_h.size() != nc)
-      return set_err(c, HPCGX_ESTATE, "level " + std::to_string(l) + " has no f2c operator");
+      return set_err(c, HPCGX_ESTATE, "level " + std::to_string(l) + " has no f2c operator for the " +
+                                          std::to_string((long long)nc) + " rows of level " + std::to_string(l + 1));
     std::vector<int> inv(F.lay.npad, -1);
     for (int64_t k = 0; k < nc; ++k) {
       const int f = F.f2c_h[k];
       if (f < 0 || f >= F.lay.n) return set_err(c, HPCGX_EINVAL, "f2c out of range");
       const int pf = F.perm_h[f];
-      if (pf < F.lay.start[ib] || pf >= F.lay.start[ib] + F.lay.count[ib] || inv[pf] >= 0)
-        return set_err(c, HPCGX_EINVAL, "f2c must map coarse rows one-to-one onto fine rows with even coordinates");
+      if (inv[pf] >= 0) return set_err(c, HPCGX_EINVAL, "f2c must map coarse rows one-to-one onto fine rows");
       inv[pf] = (int)k;
     }
-    std::vector<int> inj(nc);
-    for (int64_t t = 0; t < nc; ++t) inj[t] = Cc.perm_h[inv[F.lay.start[ib] + t]];
+    // Stencil layouts whose injected rows are exactly the class-0 block
+    // transfer over that block; otherwise over the list of injected rows.
+    const int ib = F.lay.inj;
+    bool block = !F.lay.general && !Cc.lay.general && F.lay.count[ib] == nc;
+    for (int64_t t = 0; block && t < nc; ++t) block = inv[F.lay.start[ib] + t] >= 0;
+    std::vector<int> inj(nc), frow;
+    if (block) {
+      for (int64_t t = 0; t < nc; ++t) inj[t] = Cc.perm_h[inv[F.lay.start[ib] + t]];
+    } else {
+      // ascending fine row, so a transfer launch walks the fine level in order
+      frow.reserve(nc);
+      for (int64_t p = 0; p < F.lay.npad; ++p)
+        if (inv[p] >= 0) frow.push_back((int)p);
+      for (int64_t t = 0; t < nc; ++t) inj[t] = Cc.perm_h[inv[frow[t]]];
+    }
+    F.ninj = nc;
     if ((rc = dalloc(c, (void **)&F.inj, nc * sizeof(int), false))) return rc;
     HPCGX_HIP(c, hipMemcpy(F.inj, inj.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    if (!block) {
+      if ((rc = dalloc(c, (void **)&F.frow, nc * sizeof(int), false))) return rc;
+      HPCGX_HIP(c, hipMemcpy(F.frow, frow.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    }
   }
   for (int l = 1; l < nl; ++l) {
     Level &L = c->levels[l];
@@ -403,11 +579,16 @@ int ensure_ready(Ctx *c) {
   }
   const int64_t np = c->levels[0].lay.npad;
   if ((rc = ensure_reduce(c, std::max<int64_t>((np + kBlock - 1) / kBlock, chunk_blocks(np))))) return rc;
-  if (!c->cg_r) {
-    if ((rc = dalloc(c, (void **)&c->cg_r, np * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, (void **)&c->cg_z, np * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, (void **)&c->cg_p, np * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, (void **)&c->cg_Ap, np * sizeof(double)))) return rc;
+  if (c->cg_len < np) {
+    HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+    for (double **b : {&c->cg_r, &c->cg_z, &c->cg_p, &c->cg_Ap}) {
+      dfree(c, *b, c->cg_len * sizeof(double));
+      *b = nullptr;
+    }
+    c->cg_len = 0;
+    for (double **b : {&c->cg_r, &c->cg_z, &c->cg_p, &c->cg_Ap})
+      if ((rc = dalloc(c, (void **)b, np * sizeof(double)))) return rc;
+    c->cg_len = np;
   }
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   c->ready = true;
@@ -422,27 +603,36 @@ __global__ void k_to_f32(const double *__restrict__ v, float *__restrict__ o, in
   if (i < m) o[i] = (float)v[i];
 }
 
+// Address of the diagonal of row p: slot 13 of the stencil layout, dslot[p]
+// of a general level.
+__device__ __forceinline__ size_t diag_at(const uint8_t *dslot, int64_t p) {
+  return ell_base(p) + (size_t)(dslot ? dslot[p] : kDiagSlot) * kSlice;
+}
+
 // dinv = 1/sqrt(a_ii) per row (padding: a_ii = 1)
-__global__ void k_dinv(const double *__restrict__ vals, float *__restrict__ dinv, int64_t npad) {
+__global__ void k_dinv(const double *__restrict__ vals, const uint8_t *__restrict__ dslot,
+                       float *__restrict__ dinv, int64_t npad) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= npad) return;
-  dinv[p] = (float)(1.0 / sqrt(vals[ell_base(p) + kDiagSlot * kSlice]));
+  dinv[p] = (float)(1.0 / sqrt(vals[diag_at(dslot, p)]));
 }
 
 // ahat_ij = a_ij * dinv_i * dinv_j in fp64, rounded to fp32 then fp16; diagonal 1
 __global__ void k_to_f16_scaled(const int *__restrict__ cols, const double *__restrict__ vals,
-                                __half *__restrict__ v16, int64_t npad) {
+                                const uint8_t *__restrict__ dslot, __half *__restrict__ v16,
+                                int64_t npad) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= npad) return;
   const size_t b = ell_base(p);
-  const double di = 1.0 / sqrt(vals[b + kDiagSlot * kSlice]);
+  const int ds = dslot ? dslot[p] : kDiagSlot;
+  const double di = 1.0 / sqrt(vals[b + ds * kSlice]);
   for (int s = 0; s < kSlots; ++s) {
     double a;
-    if (s == kDiagSlot) {
+    if (s == ds) {
       a = 1.0;
     } else {
       const int cj = cols[b + s * kSlice];
-      const double dj = 1.0 / sqrt(vals[ell_base(cj) + kDiagSlot * kSlice]);
+      const double dj = 1.0 / sqrt(vals[diag_at(dslot, cj)]);
       a = vals[b + s * kSlice] * di * dj;
     }
     v16[b + s * kSlice] = __float2half_rn((float)a);
@@ -450,14 +640,74 @@ __global__ void k_to_f16_scaled(const int *__restrict__ cols, const double *__re
 }
 
 // sigma_t = sqrt(d_f / d_c) for the t-th injected row
-__global__ void k_sigma(const double *__restrict__ fvals, const double *__restrict__ cvals,
-                        const int *__restrict__ inj, float *__restrict__ sigma, int64_t row0,
-                        int64_t nc) {
+__global__ void k_sigma(const double *__restrict__ fvals, const uint8_t *__restrict__ fdslot,
+                        const double *__restrict__ cvals, const uint8_t *__restrict__ cdslot,
+                        const int *__restrict__ inj, const int *__restrict__ frow,
+                        float *__restrict__ sigma, int64_t row0, int64_t nc) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= nc) return;
-  const double df = fvals[ell_base(row0 + t) + kDiagSlot * kSlice];
-  const double dc = cvals[ell_base(inj[t]) + kDiagSlot * kSlice];
+  const int64_t f = frow ? (int64_t)frow[t] : row0 + t;
+  const double df = fvals[diag_at(fdslot, f)];
+  const double dc = cvals[diag_at(cdslot, inj[t])];
   sigma[t] = (float)sqrt(df / dc);
+}
+
+// Compact column indices, one wave per 64-row slice.  For each slot the
+// columns of the rows with a nonzero value are base + lane + d with
+// d in [-128, 127]; a zero-valued slot (neighbour outside the grid) may name
+// any row of the level, as its product is an exact zero: it takes its own row
+// when reachable, else the nearest reachable one.  A slice whose columns do
+// not fit is flagged wide and keeps its explicit indices.
+__global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__ cols,
+                                                         const double *__restrict__ vals,
+                                                         int *__restrict__ cbase, int8_t *__restrict__ cdelta,
+                                                         uint8_t *__restrict__ cwide, int64_t nslices, int64_t npad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t slice = (int64_t)blockIdx.x * (kBlock / kSlice) + (threadIdx.x >> 6);
+  if (slice >= nslices) return;
+  const int64_t row = slice * kSlice + lane;
+  const size_t b = (size_t)slice * (kSlots * kSlice) + lane;
+  bool wide = false;
+  for (int s = 0; s < kSlots; ++s) {
+    const int col = cols[b + s * kSlice];
+    const bool live = vals[b + s * kSlice] != 0.0;
+    int lo = live ? col - lane : INT_MAX, hi = live ? col - lane : INT_MIN;
+    for (int m = 32; m >= 1; m >>= 1) {
+      lo = min(lo, __shfl_xor(lo, m, 64));
+      hi = max(hi, __shfl_xor(hi, m, 64));
+    }
+    int64_t base;
+    if (lo == INT_MAX) {
+      base = slice * kSlice;  // no live lane: every lane names its own row
+    } else {
+      if ((int64_t)hi - lo > 255) wide = true;
+      base = (int64_t)lo + (((int64_t)hi - lo + 1) >> 1);
+    }
+    int64_t target = live ? (int64_t)col : row;
+    const int64_t rlo = max<int64_t>(0, base + lane - 128), rhi = min<int64_t>(npad - 1, base + lane + 127);
+    if (rlo > rhi) wide = true;
+    if (!live) target = min(max(target, rlo), rhi);
+    const int64_t d = target - base - lane;
+    if (d < -128 || d > 127) wide = true;
+    cdelta[b + s * kSlice] = (int8_t)(wide ? 0 : d);
+    if (lane == 0) cbase[slice * kSlots + s] = (int)base;
+  }
+  // wide is per lane so far; the slice is wide if any lane is
+  const bool any = __any(wide);
+  if (lane == 0) cwide[slice] = any ? 1 : 0;
+}
+
+static int build_compact_cols(Ctx *c, Level &L) {
+  if (L.cbase) return HPCGX_OK;
+  const int64_t ns = L.lay.npad / kSlice;
+  int rc;
+  if ((rc = dalloc(c, (void **)&L.cbase, (size_t)ns * kSlots * sizeof(int), false))) return rc;
+  if ((rc = dalloc(c, (void **)&L.cdelta, (size_t)L.lay.npad * kSlots * sizeof(int8_t), false))) return rc;
+  if ((rc = dalloc(c, (void **)&L.cwide, (size_t)ns, false))) return rc;
+  const int64_t grid = (ns + (kBlock / kSlice) - 1) / (kBlock / kSlice);
+  k_compact_cols<<<grid, kBlock, 0, c->stream>>>(L.cols, L.vals, L.cbase, L.cdelta, L.cwide, ns, L.lay.npad);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
 }
 
 static int build_precision(Ctx *c, int prec) {
@@ -467,6 +717,7 @@ static int build_precision(Ctx *c, int prec) {
   for (int l = 0; l < nl; ++l) {
     Level &L = c->levels[l];
     const int64_t m = L.lay.npad * kSlots;
+    if ((rc = build_compact_cols(c, L))) return rc;
     if (prec == HPCGX_PREC_FP32) {
       if (!L.vals32 && (rc = dalloc(c, (void **)&L.vals32, m * sizeof(float), false))) return rc;
       k_to_f32<<<nblk(m), kBlock, 0, c->stream>>>(L.vals, L.vals32, m);
@@ -474,8 +725,9 @@ static int build_precision(Ctx *c, int prec) {
     } else {
       if (!L.vals16 && (rc = dalloc(c, (void **)&L.vals16, m * sizeof(__half), false))) return rc;
       if (!L.dinv && (rc = dalloc(c, (void **)&L.dinv, L.lay.npad * sizeof(float), false))) return rc;
-      k_to_f16_scaled<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, L.vals16, L.lay.npad);
-      k_dinv<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.vals, L.dinv, L.lay.npad);
+      k_to_f16_scaled<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, L.dslot, L.vals16,
+                                                                  L.lay.npad);
+      k_dinv<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.vals, L.dslot, L.dinv, L.lay.npad);
       HPCGX_CHECK_LAUNCH(c);
     }
     if (!L.r32 && l > 0 && (rc = dalloc(c, (void **)&L.r32, L.lay.npad * sizeof(float)))) return rc;
@@ -484,9 +736,10 @@ static int build_precision(Ctx *c, int prec) {
   if (prec == HPCGX_PREC_FP16) {
     for (int l = 0; l + 1 < nl; ++l) {
       Level &F = c->levels[l];
-      const int64_t nc = F.lay.count[F.lay.inj];
+      const Level &Cc = c->levels[l + 1];
+      const int64_t nc = F.ninj;
       if (!F.sigma && (rc = dalloc(c, (void **)&F.sigma, nc * sizeof(float), false))) return rc;
-      k_sigma<<<nblk(nc), kBlock, 0, c->stream>>>(F.vals, c->levels[l + 1].vals, F.inj, F.sigma,
+      k_sigma<<<nblk(nc), kBlock, 0, c->stream>>>(F.vals, F.dslot, Cc.vals, Cc.dslot, F.inj, F.frow, F.sigma,
                                                   F.lay.start[F.lay.inj], nc);
       HPCGX_CHECK_LAUNCH(c);
     }
@@ -494,23 +747,32 @@ static int build_precision(Ctx *c, int prec) {
     if (!L0.r16 && (rc = dalloc(c, (void **)&L0.r16, L0.lay.npad * sizeof(float)))) return rc;
   }
   const int64_t np = c->levels[0].lay.npad;
-  if (!c->cg_r32 && (rc = dalloc(c, (void **)&c->cg_r32, np * sizeof(float)))) return rc;
-  if (!c->cg_z32 && (rc = dalloc(c, (void **)&c->cg_z32, np * sizeof(float)))) return rc;
+  if (c->cg32_len < np) {
+    HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+    for (float **b : {&c->cg_r32, &c->cg_z32}) {
+      dfree(c, *b, c->cg32_len * sizeof(float));
+      *b = nullptr;
+    }
+    c->cg32_len = 0;
+    for (float **b : {&c->cg_r32, &c->cg_z32})
+      if ((rc = dalloc(c, (void **)b, np * sizeof(float)))) return rc;
+    c->cg32_len = np;
+  }
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   c->prepared |= (prec == HPCGX_PREC_FP32) ? 1 : 2;
   return HPCGX_OK;
 }
 
 // diagonal scatter / gather (natural order staging <-> colour-order ELL)
-__global__ void k_set_diag(double *__restrict__ vals, const int *__restrict__ perm,
-                           const double *__restrict__ d, int64_t n) {
+__global__ void k_set_diag(double *__restrict__ vals, const uint8_t *__restrict__ dslot,
+                           const int *__restrict__ perm, const double *__restrict__ d, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) vals[ell_base(perm[i]) + kDiagSlot * kSlice] = d[i];
+  if (i < n) vals[diag_at(dslot, perm[i])] = d[i];
 }
-__global__ void k_get_diag(const double *__restrict__ vals, const int *__restrict__ perm,
-                           double *__restrict__ d, int64_t n) {
+__global__ void k_get_diag(const double *__restrict__ vals, const uint8_t *__restrict__ dslot,
+                           const int *__restrict__ perm, double *__restrict__ d, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) d[i] = vals[ell_base(perm[i]) + kDiagSlot * kSlice];
+  if (i < n) d[i] = vals[diag_at(dslot, perm[i])];
 }
 
 }  // namespace hpcgx
@@ -599,14 +861,26 @@ int hpcgx_level_upload_packed(hpcgx_ctx *c, int level, int nx, int ny, int nz, c
                               const int *cols, const double *vals, const int *f2c) {
   if (!c || !nnz_row || !cols || !vals) return set_err(c, HPCGX_EINVAL, "level_upload_packed: null argument");
   HPCGX_HIP(c, hipSetDevice(c->device));
-  return upload_level(c, level, nx, ny, nz, PackedRows{nnz_row, cols, vals}, f2c);
+  int64_t nc = 0;
+  if ((nc = grid_coarse_rows(c, nx, ny, nz, f2c)) < 0) return (int)nc;
+  return upload_level(c, level, nx, ny, nz, PackedRows{nnz_row, cols, vals}, f2c, nc, true);
+}
+
+int hpcgx_level_upload_rows(hpcgx_ctx *c, int level, int64_t n, const uint8_t *nnz_row, const int *cols,
+                            const double *vals, const int *f2c, int64_t n_coarse) {
+  if (!c || !nnz_row || !cols || !vals) return set_err(c, HPCGX_EINVAL, "level_upload_rows: null argument");
+  if (n < 1 || n >= ((int64_t)1 << 31)) return set_err(c, HPCGX_EINVAL, "level_upload_rows: bad row count");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  return upload_level(c, level, (int)n, 1, 1, PackedRows{nnz_row, cols, vals}, f2c, f2c ? n_coarse : 0, false);
 }
 
 int hpcgx_level_upload(hpcgx_ctx *c, int level, int nx, int ny, int nz, const char *nonzerosInRow,
                        const int *const *mtxIndL, const double *const *matrixValues, const int *f2c) {
   if (!c || !nonzerosInRow || !mtxIndL || !matrixValues) return set_err(c, HPCGX_EINVAL, "level_upload: null argument");
   HPCGX_HIP(c, hipSetDevice(c->device));
-  return upload_level(c, level, nx, ny, nz, PointerRows{nonzerosInRow, mtxIndL, matrixValues}, f2c);
+  int64_t nc = 0;
+  if ((nc = grid_coarse_rows(c, nx, ny, nz, f2c)) < 0) return (int)nc;
+  return upload_level(c, level, nx, ny, nz, PointerRows{nonzerosInRow, mtxIndL, matrixValues}, f2c, nc, true);
 }
 
 int hpcgx_problem_generate(hpcgx_ctx *c, int nx, int ny, int nz, int levels, const double *kappa) {
@@ -664,6 +938,7 @@ int hpcgx_problem_generate(hpcgx_ctx *c, int nx, int ny, int nz, int levels, con
     Level &F = c->levels[l];
     const Level &Cc = c->levels[l + 1];
     if ((rc = dalloc(c, (void **)&F.inj, Cc.lay.n * sizeof(int), false))) return rc;
+    F.ninj = Cc.lay.n;
     k_inj_generated<<<nblk(Cc.lay.n), kBlock, 0, c->stream>>>(Cc.ldev, F.inj, Cc.lay.n);
     HPCGX_CHECK_LAUNCH(c);
   }
@@ -697,6 +972,23 @@ int hpcgx_level_info(const hpcgx_ctx *c, int level, int64_t *out) {
   return HPCGX_OK;
 }
 
+int hpcgx_level_colouring(const hpcgx_ctx *c, int level, int *ncolours, int *general, int *colour) {
+  if (!c || level < 0 || level >= (int)c->levels.size()) return HPCGX_EINVAL;
+  const Level &L = c->levels[level];
+  if (ncolours) *ncolours = L.lay.ncol;
+  if (general) *general = L.lay.general ? 1 : 0;
+  if (colour && L.lay.general) {
+    for (int k = 0; k < L.lay.ncol; ++k)
+      for (int64_t p = L.lay.start[k]; p < L.lay.start[k + 1]; ++p)
+        if (L.iperm_h[p] >= 0) colour[L.iperm_h[p]] = k;
+  } else if (colour) {
+    const int nx = L.lay.nx, ny = L.lay.ny;
+    for (int64_t i = 0; i < L.lay.n; ++i)
+      colour[i] = L.lay.pos[parity_class((int)(i % nx), (int)((i / nx) % ny), (int)(i / ((int64_t)nx * ny)))];
+  }
+  return HPCGX_OK;
+}
+
 int hpcgx_level_set_diagonal(hpcgx_ctx *c, int level, const double *diag) {
   if (!c || !diag || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "set_diagonal: bad argument");
   HPCGX_HIP(c, hipSetDevice(c->device));
@@ -704,7 +996,7 @@ int hpcgx_level_set_diagonal(hpcgx_ctx *c, int level, const double *diag) {
   int rc = ensure_stage(c, L.lay.n);
   if (rc) return rc;
   HPCGX_HIP(c, hipMemcpyAsync(c->stage, diag, L.lay.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  k_set_diag<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(L.vals, L.perm, c->stage, L.lay.n);
+  k_set_diag<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(L.vals, L.dslot, L.perm, c->stage, L.lay.n);
   HPCGX_CHECK_LAUNCH(c);
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   const int prepared = c->prepared;
@@ -719,7 +1011,7 @@ int hpcgx_level_get_diagonal(hpcgx_ctx *c, int level, double *diag) {
   Level &L = c->levels[level];
   int rc = ensure_stage(c, L.lay.n);
   if (rc) return rc;
-  k_get_diag<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(L.vals, L.perm, c->stage, L.lay.n);
+  k_get_diag<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(L.vals, L.dslot, L.perm, c->stage, L.lay.n);
   HPCGX_CHECK_LAUNCH(c);
   HPCGX_HIP(c, hipMemcpyAsync(diag, c->stage, L.lay.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
@@ -834,8 +1126,12 @@ int hpcgx_level_export_packed(hpcgx_ctx *c, int level, uint8_t *nnz_row, int *co
     const int ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
     int k = 0;
     for (int s = 0; s < kSlots; ++s) {
-      const int jx = ix + s % 3 - 1, jy = iy + (s / 3) % 3 - 1, jz = iz + s / 9 - 1;
-      if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) continue;
+      if (L.lay.general) {
+        if (s >= L.rowlen_h[i]) break;
+      } else {
+        const int jx = ix + s % 3 - 1, jy = iy + (s / 3) % 3 - 1, jz = iz + s / 9 - 1;
+        if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) continue;
+      }
       const int pc = hc[b + s * kSlice];
       cols[(int64_t)i * kSlots + k] = pc >= 0 && pc < L.lay.npad ? ip[pc] : -2;
       vals[(int64_t)i * kSlots + k] = hv[b + s * kSlice];

@@ -32,7 +32,7 @@ extern "C" {
 
 #define HPCGX_OK 0
 #define HPCGX_EINVAL (-1)      /* bad argument */
-#define HPCGX_ENOTSTENCIL (-2) /* matrix is not a 27-point stencil on the stated grid */
+#define HPCGX_ENOTSTENCIL (-2) /* reserved: no longer returned (any rows upload, see hpcgx_level_upload) */
 #define HPCGX_EHIP (-3)        /* a HIP runtime call failed */
 #define HPCGX_ENOMEM (-4)      /* device or host allocation failed */
 #define HPCGX_ESTATE (-5)      /* call out of order (e.g. level missing) */
@@ -80,7 +80,14 @@ int hpcgx_set_colour_order(hpcgx_ctx *ctx, const int *order);
 int hpcgx_level_upload(hpcgx_ctx *ctx, int level, int nx, int ny, int nz, const char *nonzerosInRow,
                        const int *const *mtxIndL, const double *const *matrixValues, const int *f2c);
 /* Same, from packed arrays: row i's entries at cols[i*27 + j], vals[i*27 + j]
- * for j < nnz_row[i], ascending column. */
+ * for j < nnz_row[i].
+ * Both uploads accept any rows, as the reference requires of every
+ * optimisation (GenerateProblem.cpp:47-49): rows that are the 27-point
+ * stencil of the stated grid (ascending columns) get the parity-class layout;
+ * other rows (any pattern, <= 27 entries, a diagonal) get a general layout
+ * coloured greedily like the reference's example (OptimizeProblem.cpp:43-79),
+ * summed in the row's own entry order.  f2c then has (nx/2)(ny/2)(nz/2)
+ * entries, the reference's coarse grid (GenerateCoarseProblem.cpp:50-53). */
 int hpcgx_level_upload_packed(hpcgx_ctx *ctx, int level, int nx, int ny, int nz,
                               const uint8_t *nnz_row, const int *cols, const double *vals,
                               const int *f2c);
@@ -91,9 +98,19 @@ int hpcgx_level_upload_packed(hpcgx_ctx *ctx, int level, int nx, int ny, int nz,
  * (host pointer, nx*ny*nz) for the variable-coefficient operator
  * a_ij = -(k_i+k_j)/2, a_ii = sum_j (k_i+k_j)/2 + (26-n_i) k_i; coarse
  * levels take kappa by injection. */
+/* General rows without a grid: n rows packed as above, f2c the injection
+ * onto the n_coarse rows of level+1 (one-to-one) or NULL.  Always the general
+ * layout.  Replaces OptimizeProblem.cpp:37 for matrices the caller builds. */
+int hpcgx_level_upload_rows(hpcgx_ctx *ctx, int level, int64_t n, const uint8_t *nnz_row, const int *cols,
+                            const double *vals, const int *f2c, int64_t n_coarse);
 int hpcgx_problem_generate(hpcgx_ctx *ctx, int nx, int ny, int nz, int levels, const double *kappa);
 /* out[0..7]: rows, padded rows, nonzeros, nx, ny, nz, stored slots, levels */
 int hpcgx_level_info(const hpcgx_ctx *ctx, int level, int64_t *out);
+/* The colouring a level is swept in (OptimizeProblem.cpp:42-97): number of
+ * colours, whether the level has the general layout, and (if colour is not
+ * NULL) the colour of every row in natural order; colours are swept
+ * 0..ncolours-1 forward and back down to 0. */
+int hpcgx_level_colouring(const hpcgx_ctx *ctx, int level, int *ncolours, int *general, int *colour);
 /* Replaces the diagonal of a level (ReplaceMatrixDiagonal, SparseMatrix.hpp:139-145);
  * diag is host, natural order.  Lower-precision copies are rebuilt. */
 int hpcgx_level_set_diagonal(hpcgx_ctx *ctx, int level, const double *diag);

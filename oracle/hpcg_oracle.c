@@ -27,7 +27,9 @@ struct orc_level {
   int *f2c;               /* [coarse->n] fine row injected into coarse row */
   int order[8];           /* parity class swept k-th (forward) */
   int *crow;              /* rows grouped by sweep position */
-  int cstart[9];
+  int *cstart;            /* [ncol+1] first entry of each colour in crow */
+  int ncol;               /* colours: 8 parity classes, or a general colouring */
+  int general;            /* built from arbitrary rows (orc_create_rows) */
   /* mixed-precision copies (built on first use) */
   float *vals32;
   uint16_t *vals16; /* diagonally scaled, fp16 bits */
@@ -46,6 +48,9 @@ static int parity_class(const orc_level *L, int i) {
 
 /* Rows grouped by sweep position, ascending inside a colour. */
 static void build_colour_lists(orc_level *L) {
+  free(L->cstart);
+  L->cstart = (int *)malloc(9 * sizeof(int));
+  L->ncol = 8;
   int pos_of_class[8];
   for (int k = 0; k < 8; ++k) pos_of_class[L->order[k]] = k;
   int count[8] = {0};
@@ -155,7 +160,7 @@ orc_level *orc_create(int nx, int ny, int nz, int levels, const double *kappa) {
 void orc_destroy(orc_level *L) {
   if (!L) return;
   orc_destroy(L->coarse);
-  free(L->nnz_row); free(L->cols); free(L->vals); free(L->diag_slot); free(L->crow);
+  free(L->nnz_row); free(L->cols); free(L->vals); free(L->diag_slot); free(L->crow); free(L->cstart);
   free(L->kappa); free(L->f2c); free(L->Axf); free(L->rc); free(L->xc);
   free(L->Axf32); free(L->rc32); free(L->xc32);
   free(L->vals32); free(L->vals16); free(L->dinv); free(L->sigma);
@@ -194,6 +199,7 @@ void orc_set_diagonal(orc_level *L, const double *d) {
 
 void orc_set_colour_order(orc_level *L, const int order[8]) {
   for (; L; L = L->coarse) {
+    if (L->general) continue;
     memcpy(L->order, order, 8 * sizeof(int));
     build_colour_lists(L);
   }
@@ -218,6 +224,155 @@ void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours) {
     colour[i] = pick;
   }
   *ncolours = total;
+}
+
+/* First-fit greedy colouring in row order over the symmetrised pattern:
+ * row i avoids the colours of every j < i with a_ij != 0 or a_ji != 0.  On a
+ * structurally symmetric matrix this is exactly orc_colour_greedy (the
+ * reference's OptimizeProblem.cpp:43-79 example, which looks at a_ij only);
+ * the transpose term keeps colours independent for unsymmetric patterns. */
+void orc_colour_general(const orc_level *L, int *colour, int *ncolours) {
+  const int n = L->n;
+  /* lower neighbours from the transpose: rows j < i whose row holds i */
+  int *tcnt = (int *)calloc((size_t)n + 1, sizeof(int));
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < L->nnz_row[j]; ++k) {
+      const int c = L->cols[(int64_t)j * NSLOT + k];
+      if (c > j) tcnt[c + 1]++;
+    }
+  for (int i = 0; i < n; ++i) tcnt[i + 1] += tcnt[i];
+  int *tj = (int *)malloc((size_t)(tcnt[n] > 0 ? tcnt[n] : 1) * sizeof(int));
+  int *fill = (int *)malloc((size_t)n * sizeof(int));
+  memcpy(fill, tcnt, (size_t)n * sizeof(int));
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < L->nnz_row[j]; ++k) {
+      const int c = L->cols[(int64_t)j * NSLOT + k];
+      if (c > j) tj[fill[c]++] = j;
+    }
+  int *mark = (int *)malloc(((size_t)n + 1) * sizeof(int));
+  for (int i = 0; i <= n; ++i) mark[i] = -1;
+  int total = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < L->nnz_row[i]; ++k) {
+      const int c = L->cols[(int64_t)i * NSLOT + k];
+      if (c < i) mark[colour[c]] = i;
+    }
+    for (int t = tcnt[i]; t < tcnt[i + 1]; ++t) mark[colour[tj[t]]] = i;
+    int pick = 0;
+    while (mark[pick] == i) ++pick;
+    colour[i] = pick;
+    if (pick + 1 > total) total = pick + 1;
+  }
+  *ncolours = total;
+  free(tcnt); free(tj); free(fill); free(mark);
+}
+
+/* Rows grouped by colour (colours swept in ascending order, rows ascending
+ * inside a colour).  Returns -1 unless every colour is independent: no row
+ * has an off-diagonal entry in a row of its own colour. */
+int orc_set_colouring(orc_level *L, const int *colour, int ncolours) {
+  const int n = L->n;
+  if (ncolours < 1) return -1;
+  for (int i = 0; i < n; ++i) {
+    if (colour[i] < 0 || colour[i] >= ncolours) return -1;
+    for (int k = 0; k < L->nnz_row[i]; ++k) {
+      const int c = L->cols[(int64_t)i * NSLOT + k];
+      if (c != i && colour[c] == colour[i]) return -1;
+    }
+  }
+  free(L->cstart);
+  L->cstart = (int *)calloc((size_t)ncolours + 1, sizeof(int));
+  L->ncol = ncolours;
+  for (int i = 0; i < n; ++i) L->cstart[colour[i] + 1]++;
+  for (int k = 0; k < ncolours; ++k) L->cstart[k + 1] += L->cstart[k];
+  int *fill = (int *)malloc((size_t)ncolours * sizeof(int));
+  memcpy(fill, L->cstart, (size_t)ncolours * sizeof(int));
+  for (int i = 0; i < n; ++i) L->crow[fill[colour[i]]++] = i;
+  free(fill);
+  return 0;
+}
+
+int orc_ncolours(const orc_level *L) { return L->ncol; }
+const int *orc_colour_rows(const orc_level *L) { return L->crow; }
+const int *orc_colour_starts(const orc_level *L) { return L->cstart; }
+
+/* A level from arbitrary rows held as the reference holds them
+ * (SparseMatrix.hpp:44-52): row i has nnz_row[i] <= 27 entries cols/vals at
+ * i*27, in the row's own order, one of them the diagonal (its first
+ * occurrence is matrixDiagonal, GenerateProblem_ref.cpp:163).  Coloured with
+ * orc_colour_general.  NULL if a row is empty, too long, has a column out of
+ * range or no diagonal. */
+orc_level *orc_create_rows(int n, const unsigned char *nnz_row, const int *cols, const double *vals) {
+  if (n < 1) return NULL;
+  for (int i = 0; i < n; ++i) {
+    if (nnz_row[i] < 1 || nnz_row[i] > NSLOT) return NULL;
+    int diag = 0;
+    for (int k = 0; k < nnz_row[i]; ++k) {
+      const int c = cols[(int64_t)i * NSLOT + k];
+      if (c < 0 || c >= n) return NULL;
+      if (c == i) diag = 1;
+    }
+    if (!diag) return NULL;
+  }
+  orc_level *L = (orc_level *)calloc(1, sizeof(orc_level));
+  L->nx = n; L->ny = 1; L->nz = 1;
+  L->n = n;
+  L->general = 1;
+  L->nnz_row = (unsigned char *)malloc((size_t)n);
+  L->cols = (int *)malloc((size_t)n * NSLOT * sizeof(int));
+  L->vals = (double *)malloc((size_t)n * NSLOT * sizeof(double));
+  L->diag_slot = (int *)malloc((size_t)n * sizeof(int));
+  L->crow = (int *)malloc((size_t)n * sizeof(int));
+  int64_t nnz = 0;
+  for (int i = 0; i < n; ++i) {
+    L->nnz_row[i] = nnz_row[i];
+    nnz += nnz_row[i];
+    L->diag_slot[i] = -1;
+    for (int k = 0; k < NSLOT; ++k) {
+      const int64_t e = (int64_t)i * NSLOT + k;
+      if (k < nnz_row[i]) {
+        L->cols[e] = cols[e];
+        L->vals[e] = vals[e];
+        if (cols[e] == i && L->diag_slot[i] < 0) L->diag_slot[i] = k;
+      } else {
+        L->cols[e] = i;
+        L->vals[e] = 0.0;
+      }
+    }
+  }
+  L->nnz = nnz;
+  for (int k = 0; k < 8; ++k) L->order[k] = k;
+  int *colour = (int *)malloc((size_t)n * sizeof(int));
+  int nc = 0;
+  orc_colour_general(L, colour, &nc);
+  orc_set_colouring(L, colour, nc);
+  free(colour);
+  L->Axf = (double *)calloc((size_t)n, sizeof(double));
+  L->Axf32 = (float *)calloc((size_t)n, sizeof(float));
+  return L;
+}
+
+/* Makes coarse the next level of fine with the injection f2c[n_coarse]
+ * (MGData.hpp:56-63: the coarse operator and f2cOperator).  fine takes
+ * ownership of coarse.  -1 if f2c is out of range or not one-to-one (the
+ * prolongation x[f2c[i]] += xc[i] scatters). */
+int orc_link(orc_level *fine, orc_level *coarse, const int *f2c) {
+  if (fine->coarse || !coarse) return -1;
+  const int nc = coarse->n;
+  char *seen = (char *)calloc((size_t)fine->n, 1);
+  for (int i = 0; i < nc; ++i) {
+    if (f2c[i] < 0 || f2c[i] >= fine->n || seen[f2c[i]]) { free(seen); return -1; }
+    seen[f2c[i]] = 1;
+  }
+  free(seen);
+  fine->f2c = (int *)malloc((size_t)nc * sizeof(int));
+  memcpy(fine->f2c, f2c, (size_t)nc * sizeof(int));
+  fine->coarse = coarse;
+  fine->rc = (double *)calloc((size_t)nc, sizeof(double));
+  fine->xc = (double *)calloc((size_t)nc, sizeof(double));
+  fine->rc32 = (float *)calloc((size_t)nc, sizeof(float));
+  fine->xc32 = (float *)calloc((size_t)nc, sizeof(float));
+  return 0;
 }
 
 /* y = A x, ComputeSPMV_ref.cpp:61-70: per row, left-to-right over the
@@ -261,11 +416,11 @@ void orc_symgs(const orc_level *L, const double *r, double *x, int mode) {
     for (int i = L->n - 1; i >= 0; --i) gs_row(L, i, r, x);
     return;
   }
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < L->ncol; ++k) {
 #pragma omp parallel for schedule(static)
     for (int t = L->cstart[k]; t < L->cstart[k + 1]; ++t) gs_row(L, L->crow[t], r, x);
   }
-  for (int k = mode == 2 ? 6 : 7; k >= 0; --k) {
+  for (int k = mode == 2 ? L->ncol - 2 : L->ncol - 1; k >= 0; --k) {
 #pragma omp parallel for schedule(static)
     for (int t = L->cstart[k]; t < L->cstart[k + 1]; ++t) gs_row(L, L->crow[t], r, x);
   }
@@ -467,11 +622,11 @@ static inline void gs_row32(const orc_level *L, int i, const float *r, float *x,
 }
 
 static void symgs32(const orc_level *L, const float *r, float *x, int half) {
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < L->ncol; ++k) {
 #pragma omp parallel for schedule(static)
     for (int t = L->cstart[k]; t < L->cstart[k + 1]; ++t) gs_row32(L, L->crow[t], r, x, half);
   }
-  for (int k = 6; k >= 0; --k) {
+  for (int k = L->ncol - 2; k >= 0; --k) {
 #pragma omp parallel for schedule(static)
     for (int t = L->cstart[k]; t < L->cstart[k + 1]; ++t) gs_row32(L, L->crow[t], r, x, half);
   }

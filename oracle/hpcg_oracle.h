@@ -68,6 +68,20 @@ void orc_set_diagonal(orc_level *L, const double *d);
 void orc_set_colour_order(orc_level *L, const int order[8]);
 void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours);
 
+/* General (unstructured) levels.  orc_create_rows builds one level from
+ * arbitrary rows (<= 27 entries each, row order kept, one diagonal) and
+ * colours it with orc_colour_general; orc_link attaches a coarse level with
+ * an injection f2c.  orc_set_colouring replaces the colouring of one level
+ * (colours swept in ascending order); -1 if two rows of one colour are
+ * coupled. */
+orc_level *orc_create_rows(int n, const unsigned char *nnz_row, const int *cols, const double *vals);
+int orc_link(orc_level *fine, orc_level *coarse, const int *f2c);
+void orc_colour_general(const orc_level *L, int *colour, int *ncolours);
+int orc_set_colouring(orc_level *L, const int *colour, int ncolours);
+int orc_ncolours(const orc_level *L);
+const int *orc_colour_rows(const orc_level *L);   /* rows grouped by colour */
+const int *orc_colour_starts(const orc_level *L); /* [ncolours+1] */
+
 /* Kernels (natural row order, full-length vectors).  mode selects the
  * Gauss-Seidel ordering: 0 natural (the reference), 1 coloured, 2 coloured
  * with the redundant backward step of the last colour elided (the device's). */

@@ -70,6 +70,15 @@ def lib():
         L.orc_cg_mixed.argtypes = [vp, _dp, _dp, C.c_int, C.c_double, C.c_int,
                                    C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
         L.orc_kappa_grad.argtypes = [vp, _dp, _dp, _dp]
+        L.orc_create_rows.restype = vp
+        L.orc_create_rows.argtypes = [C.c_int, _u8p, _ip, _dp]
+        L.orc_link.restype = C.c_int
+        L.orc_link.argtypes = [vp, vp, _ip]
+        L.orc_colour_general.argtypes = [vp, _ip, C.POINTER(C.c_int)]
+        L.orc_set_colouring.restype = C.c_int
+        L.orc_set_colouring.argtypes = [vp, _ip, C.c_int]
+        L.orc_ncolours.restype = C.c_int
+        L.orc_ncolours.argtypes = [vp]
         L.orc_f32_to_f16.restype = C.c_uint16
         L.orc_f32_to_f16.argtypes = [C.c_float]
         L.orc_f16_to_f32.restype = C.c_float
@@ -161,19 +170,67 @@ class Level:
     def set_diagonal(self, d):
         lib().orc_set_diagonal(self.h, np.ascontiguousarray(d, np.float64))
 
+    def greedy_colours(self):
+        """orc_colour_greedy (the reference's example, lower neighbours only)."""
+        col = np.empty(self.n, np.int32)
+        nc = C.c_int(0)
+        lib().orc_colour_greedy(self.h, col, C.byref(nc))
+        return col, nc.value
+
+    def general_colours(self):
+        """orc_colour_general: (colour per row, number of colours)."""
+        col = np.empty(self.n, np.int32)
+        nc = C.c_int(0)
+        lib().orc_colour_general(self.h, col, C.byref(nc))
+        return col, nc.value
+
+    def set_colouring(self, colour):
+        colour = np.ascontiguousarray(colour, np.int32)
+        if lib().orc_set_colouring(self.h, colour, int(colour.max()) + 1) != 0:
+            raise ValueError("colouring couples two rows of one colour")
+
+    def ncolours(self):
+        return lib().orc_ncolours(self.h)
+
 
 class Problem(Level):
     """Oracle hierarchy: HPCG matrix (kappa None) or the kappa operator."""
 
-    def __init__(self, nx, ny, nz, levels=4, kappa=None):
+    def __init__(self, nx, ny, nz, levels=4, kappa=None, _handle=None):
         L = lib()
         self._kappa = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
         kp = None if self._kappa is None else self._kappa.ctypes.data_as(C.c_void_p)
-        h = L.orc_create(nx, ny, nz, levels, kp)
+        h = _handle or L.orc_create(nx, ny, nz, levels, kp)
         if not h:
             raise ValueError("bad grid")
         super().__init__(h, self)
         self.levels = levels
+
+    @classmethod
+    def from_rows(cls, levels):
+        """A hierarchy of general levels, fine to coarse: each entry is
+        (nnz_row[n], cols[n,27], vals[n,27], f2c) with f2c the injection onto
+        the next level (None on the coarsest).  Each level is coloured with
+        orc_colour_general."""
+        L = lib()
+        hs = []
+        for nr, cols, vals, _ in levels:
+            nr = np.ascontiguousarray(nr, np.uint8)
+            h = L.orc_create_rows(len(nr), nr, np.ascontiguousarray(cols, np.int32).reshape(-1),
+                                  np.ascontiguousarray(vals, np.float64).reshape(-1))
+            if not h:
+                for g in hs:
+                    L.orc_destroy(g)
+                raise ValueError("rows rejected (empty/long row, column out of range or no diagonal)")
+            hs.append(h)
+        for k in range(len(hs) - 2, -1, -1):
+            f2c = np.ascontiguousarray(levels[k][3], np.int32)
+            if L.orc_link(hs[k], hs[k + 1], f2c) != 0:
+                for g in hs[: k + 2]:
+                    L.orc_destroy(g)
+                raise ValueError("bad f2c")
+        n = len(levels[0][0])
+        return cls(n, 1, 1, levels=len(levels), _handle=hs[0])
 
     def __del__(self):
         try:
@@ -194,12 +251,6 @@ class Problem(Level):
 
     def set_colour_order(self, order):
         lib().orc_set_colour_order(self.h, np.ascontiguousarray(order, np.int32))
-
-    def greedy_colours(self):
-        col = np.empty(self.n, np.int32)
-        nc = C.c_int(0)
-        lib().orc_colour_greedy(self.h, col, C.byref(nc))
-        return col, nc.value
 
     def cg(self, b, x0=None, max_iter=50, tol=0.0, precond=True, mode=2, history=False):
         x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64, copy=True)
@@ -272,6 +323,8 @@ def ref(omp=False):
         R.ref_level_export.argtypes = [vp, C.c_int, _u8p, _ip, _dp, _ip, vp]
         R.ref_rhs.argtypes = [vp, _dp, _dp, _dp]
         R.ref_set_diagonal.argtypes = [vp, C.c_int, _dp]
+        R.ref_replace_rows.restype = C.c_int
+        R.ref_replace_rows.argtypes = [vp, C.c_int, _u8p, _ip, _dp, vp]
         for f in ("ref_spmv", "ref_symgs", "ref_mg"):
             getattr(R, f).restype = C.c_int
             getattr(R, f).argtypes = [vp, C.c_int, _dp, _dp]
@@ -329,6 +382,16 @@ class RefProblem:
 
     def set_diagonal(self, level, d):
         ref(self.omp).ref_set_diagonal(self.h, level, np.ascontiguousarray(d, np.float64))
+
+    def replace_rows(self, level, nnz_row, cols, vals, f2c=None):
+        """Overwrite a level's rows (same row count) and optionally its f2c."""
+        f = None if f2c is None else np.ascontiguousarray(f2c, np.int32)
+        rc = ref(self.omp).ref_replace_rows(
+            self.h, level, np.ascontiguousarray(nnz_row, np.uint8),
+            np.ascontiguousarray(cols, np.int32).reshape(-1), np.ascontiguousarray(vals, np.float64).reshape(-1),
+            None if f is None else f.ctypes.data_as(C.c_void_p))
+        if rc:
+            raise ValueError(f"ref_replace_rows failed ({rc})")
 
     def spmv(self, x, level=0):
         y = np.empty(self.info[level]["nrow"])

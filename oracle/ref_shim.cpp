@@ -122,6 +122,41 @@ int ref_level_export(void *h, int level, unsigned char *nnz_row, int *cols, doub
   return 0;
 }
 
+/* Replaces the rows of one level in place with arbitrary rows of the same
+ * count (<= 27 entries each, one of them the diagonal), writing into the
+ * reference's own per-row arrays (GenerateProblem_ref.cpp:104-110) the
+ * fields its kernels read: nonzerosInRow, mtxIndL, matrixValues,
+ * matrixDiagonal (the diagonal entry, as :160 sets it).  f2c (may be
+ * NULL) replaces the level's f2cOperator for its nrow(level+1) coarse rows.
+ * Returns nonzero if a row cannot be stored. */
+int ref_replace_rows(void *h, int level, const unsigned char *nnz_row, const int *cols, const double *vals,
+                     const int *f2c) {
+  SparseMatrix *m = level_of(static_cast<RefProblem *>(h), level);
+  if (!m) return 1;
+  const int n = m->localNumberOfRows;
+  int64_t nnz = 0;
+  for (int i = 0; i < n; ++i) {
+    const int k = nnz_row[i];
+    if (k < 1 || k > 27) return 2; /* numberOfNonzerosPerRow, GenerateProblem_ref.cpp:67 */
+    int ds = -1;
+    for (int j = 0; j < k; ++j) {
+      const int c = cols[(int64_t)i * 27 + j];
+      if (c < 0 || c >= m->localNumberOfColumns) return 3;
+      m->mtxIndL[i][j] = c;
+      m->matrixValues[i][j] = vals[(int64_t)i * 27 + j];
+      if (c == i && ds < 0) ds = j;
+    }
+    if (ds < 0) return 4;
+    m->nonzerosInRow[i] = (char)k;
+    m->matrixDiagonal[i] = &m->matrixValues[i][ds];
+    nnz += k;
+  }
+  m->localNumberOfNonzeros = nnz;
+  if (f2c && m->mgData && m->Ac)
+    std::memcpy(m->mgData->f2cOperator, f2c, sizeof(int) * m->Ac->localNumberOfRows);
+  return 0;
+}
+
 void ref_rhs(void *h, double *b, double *x, double *xexact) {
   RefProblem *p = static_cast<RefProblem *>(h);
   const int n = p->A.localNumberOfRows;

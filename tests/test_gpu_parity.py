@@ -67,13 +67,40 @@ def test_upload_equals_generate(hip):
     assert np.array_equal(ys[0], ys[1])
 
 
-def test_upload_rejects_non_stencil(hip):
+def test_upload_non_stencil_takes_general_layout(hip):
+    """A row that is not the stencil of the stated grid: the level is
+    uploaded in the general layout (not rejected) and matches the oracle's
+    general level."""
+    P = po.Problem(8, 8, 8, levels=1)
+    nr, cols, vals, _ = P.arrays()
+    nr, cols, vals = nr.copy(), cols.copy(), vals.copy()
+    a, b = 0, P.n - 1                      # opposite corners (8 entries each)
+    cols[a, nr[a]], vals[a, nr[a]] = b, -0.5
+    cols[b, nr[b]], vals[b, nr[b]] = a, -0.5
+    nr[a] += 1
+    nr[b] += 1
+    ctx = hip.Context(0)
+    ctx.upload_level(0, P.dims, nr, cols, vals, None)
+    assert ctx.colouring(0)[2]
+    G = po.Problem.from_rows([(nr, cols, vals, None)])
+    x = rng(6).random(P.n)
+    dx, dy = ctx.vec(0, x), ctx.vec(0)
+    ctx.spmv(0, dx, dy)
+    assert np.array_equal(dy.download(), G.spmv(x))
+    r = rng(7).random(P.n)
+    dr, dz = ctx.vec(0, r), ctx.vec(0)
+    ctx.mg(0, dr, dz)
+    assert np.array_equal(dz.download(), G.mg(r, mode=2))
+    ctx.close()
+
+
+def test_upload_rejects_rows_without_diagonal(hip):
     P = po.Problem(8, 8, 8, levels=1)
     nr, cols, vals, _ = P.arrays()
     cols = cols.copy()
-    cols[100, 0] = (cols[100, 0] + 50) % P.n   # a far-away column
+    cols[5, :nr[5]] = np.where(cols[5, :nr[5]] == 5, 6, cols[5, :nr[5]])
     ctx = hip.Context(0)
-    with pytest.raises(hip.HpcgxError, match="ENOTSTENCIL"):
+    with pytest.raises(hip.HpcgxError, match="no diagonal"):
         ctx.upload_level(0, P.dims, nr, cols, vals, None)
 
 
