@@ -383,6 +383,7 @@ int hpcgx_kappa_grad(hpcgx_ctx *c, const double *lam, const double *x, double *g
   int rc = ensure_ready(c);
   if (rc) return rc;
   const Level &L = c->levels[0];
+  if (L.lay.general) return set_err(c, HPCGX_EINVAL, "kappa_grad: the kappa operator is a 27-point stencil level");
   k_kappa_grad<<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.iperm, lam, x, g, L.lay.npad,
                                                             L.lay.interleave);
   HPCGX_CHECK_LAUNCH(c);

@@ -12,6 +12,8 @@
 // reference's x86-64 build.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace hpcgx {
@@ -39,6 +41,13 @@ __device__ __forceinline__ int64_t map_row(int64_t block_rows, int64_t tile) {
   if (block_rows == 0) return tile * kBlock + threadIdx.x;
   return (tile & 7) * block_rows + (tile >> 3) * kBlock + threadIdx.x;
 }
+
+// Minimum resident blocks per CU the sweep kernel is compiled for (bounds
+// its registers: 256-thread blocks, 4 SIMDs, so 8 blocks = 8 waves per SIMD).
+#ifndef HPCGX_GS_MIN_BLOCKS
+#define HPCGX_GS_MIN_BLOCKS 1
+#endif
+constexpr int kGsMinBlocks = HPCGX_GS_MIN_BLOCKS;
 
 // Matrix loads.  NT (non-temporal): the fine level's matrix is streamed once
 // per pass and is far larger than any cache, so it is kept out of the way of
@@ -77,14 +86,24 @@ struct ColIdx {
   const uint8_t *cwide;
 };
 
-template <bool CI, bool NT>
-__device__ __forceinline__ int load_col(const ColIdx &ci, int64_t row, size_t b, int s, bool wide) {
-  if constexpr (!CI) {
+// Slice of a row.  UNI: the 64 lanes of the wave hold the 64 rows of one
+// slice (launches over 64-aligned rows), so the slice is wave-uniform and the
+// per-slice metadata comes through the scalar cache.
+template <bool UNI>
+__device__ __forceinline__ int slice_of(int64_t row) {
+  if constexpr (UNI) return __builtin_amdgcn_readfirstlane((int)(row >> 6));
+  else return (int)(row >> 6);
+}
+
+// Column of (row, slot s).  CM 0: the explicit 32-bit index; CM 1: the
+// compact form base + lane + int8 delta.  A launch picks one per wave
+// (slice_wide), so each inner loop is straight-line code.
+template <int CM, bool NT>
+__device__ __forceinline__ int col_at(const ColIdx &ci, int64_t row, int slice, size_t b, int s) {
+  if constexpr (CM == 0) {
     return ld_col<NT>(ci.cols + b + s * kSlice);
   } else {
-    if (wide) return ld_col<NT>(ci.cols + b + s * kSlice);
-    const int64_t slice = row >> 6;
-    const int base = ci.cbase[slice * kSlots + s];
+    const int base = ci.cbase[(int64_t)slice * kSlots + s];
     int8_t d;
     if constexpr (NT) d = __builtin_nontemporal_load(ci.cdelta + b + s * kSlice);
     else d = ci.cdelta[b + s * kSlice];
@@ -92,10 +111,16 @@ __device__ __forceinline__ int load_col(const ColIdx &ci, int64_t row, size_t b,
   }
 }
 
-template <bool CI>
-__device__ __forceinline__ bool slice_wide(const ColIdx &ci, int64_t row) {
+template <int V>
+using cmode = std::integral_constant<int, V>;
+
+// Whether the slice keeps explicit indices; the same for the whole wave when
+// the slice is (UNI), which keeps the choice a scalar branch.
+template <bool CI, bool UNI = true>
+__device__ __forceinline__ bool slice_wide(const ColIdx &ci, int slice) {
   if constexpr (!CI) return false;
-  else return ci.cwide[row >> 6] != 0;
+  else if constexpr (UNI) return __builtin_amdgcn_readfirstlane((int)ci.cwide[slice]) != 0;
+  else return ci.cwide[slice] != 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -250,7 +275,7 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
 // profiles show the dominant launches separately from the coarse levels').
 // GEN: general level, the diagonal's slot is dslot[row] (stencil: slot 13).
 template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false>
-__global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
+__global__ __launch_bounds__(kBlock, kGsMinBlocks) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
                                                const XT *__restrict__ r, XT *__restrict__ x,
                                                int64_t row0, int64_t nrows,
                                                const uint8_t *__restrict__ dslot) {
@@ -260,17 +285,28 @@ __global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__rest
   if (t >= nrows) return;
   const int64_t row = row0 + t;
   const size_t b = ell_base(row);
-  const bool wide = slice_wide<CI>(ci, row);
+  const int slice = slice_of<true>(row);
   int ds = kDiagSlot;
   if constexpr (GEN) ds = dslot[row];
   const ACC d = ld_val<ACC, FINE>(vals + b + ds * kSlice);
-  ACC sum = (ACC)r[row];
+  auto row_sum = [&](auto cm) {
+    constexpr int CM = decltype(cm)::value;
+    ACC sum = (ACC)r[row];
 #pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    if (!((mask >> s) & 1u)) continue;
-    const int c = load_col<CI, FINE>(ci, row, b, s, wide);
-    const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
-    sum -= v * (ACC)x[c];
+    for (int s = 0; s < kSlots; ++s) {
+      if (!((mask >> s) & 1u)) continue;
+      const int c = col_at<CM, FINE>(ci, row, slice, b, s);
+      const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
+      sum -= v * (ACC)x[c];
+    }
+    return sum;
+  };
+  ACC sum;
+  if constexpr (CI) {
+    if (slice_wide<CI>(ci, slice)) sum = row_sum(cmode<0>{});
+    else sum = row_sum(cmode<1>{});
+  } else {
+    sum = row_sum(cmode<0>{});
   }
   if (ZK < 0) sum += (ACC)x[row] * d;
   x[row] = (XT)(sum / d);
@@ -291,13 +327,28 @@ __global__ __launch_bounds__(kBlock) void k_restrict(const ColIdx ci, const VT *
   if (t >= count) return;
   const int64_t row = frow ? (int64_t)frow[t] : row0 + t;
   const size_t b = ell_base(row);
-  const bool wide = slice_wide<CI>(ci, row);
-  ACC ax = 0;
+  auto row_ax = [&](auto cm, int slice) {
+    constexpr int CM = decltype(cm)::value;
+    ACC ax = 0;
 #pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    const int c = load_col<CI, true>(ci, row, b, s, wide);
-    const ACC v = ld_val<ACC>(vals + b + s * kSlice);
-    ax += v * (ACC)x[c];
+    for (int s = 0; s < kSlots; ++s) {
+      const int c = col_at<CM, true>(ci, row, slice, b, s);
+      const ACC v = ld_val<ACC>(vals + b + s * kSlice);
+      ax += v * (ACC)x[c];
+    }
+    return ax;
+  };
+  ACC ax;
+  if constexpr (CI) {
+    if (frow) {  // rows from a list: slices differ across the wave
+      const int slice = slice_of<false>(row);
+      ax = slice_wide<CI, false>(ci, slice) ? row_ax(cmode<0>{}, slice) : row_ax(cmode<1>{}, slice);
+    } else {
+      const int slice = slice_of<true>(row);
+      ax = slice_wide<CI, true>(ci, slice) ? row_ax(cmode<0>{}, slice) : row_ax(cmode<1>{}, slice);
+    }
+  } else {
+    ax = row_ax(cmode<0>{}, 0);
   }
   ACC res = (ACC)r[row] - ax;
   if (SCALE) res = res * (ACC)sigma[t];

@@ -2,10 +2,12 @@
 used for roofline fractions.
 
 Algorithmic bytes count each array element once per pass: matrix values
-(v bytes) and 4-byte column indices per stored nonzero of the rows a kernel
-touches, plus the vector elements it reads or writes (s bytes each).  ELL
-padding, the gathered re-reads of neighbour entries and metadata are not
-counted, so achieved/peak is a lower bound on how busy HBM is.
+(v bytes) and column indices (i bytes: 4 for the fp64 matrices, 1 for the
+int8 deltas of the compact fp32/fp16 copies) per stored nonzero of the rows a
+kernel touches, plus the vector elements it reads or writes (s bytes each).
+ELL padding, the gathered re-reads of neighbour entries and metadata (the
+compact copies' per-slice bases) are not counted, so achieved/peak is a lower
+bound on how busy HBM is.
 """
 
 
@@ -58,22 +60,22 @@ def colour_rows(nx, ny, nz):
     return [rx[c & 1] * ry[(c >> 1) & 1] * rz[(c >> 2) & 1] for c in range(8)]
 
 
-def kernel_bytes(kind, n, m, v=8, s=8, nc=0, mc=0):
-    """Algorithmic bytes of one call.
-    spmv: m(v+4) + 2ns            symgs (fwd+bwd): 2[m(v+4) + 3ns]
-    gs_step (one colour, full):  m(v+4) + 3ns  with n, m of that colour
-    restrict (fused):            mc(v+4) + ns_c(read r) + n_c(4 + 2s)... see below
+def kernel_bytes(kind, n, m, v=8, s=8, nc=0, mc=0, i=4):
+    """Algorithmic bytes of one call (i: column-index bytes per nonzero).
+    spmv: m(v+i) + 2ns            symgs (fwd+bwd): 2[m(v+i) + 3ns]
+    gs_step (one colour, full):  m(v+i) + 3ns  with n, m of that colour
+    restrict (fused):            mc(v+i) + ns_c(read r) + n_c(4 + 2s)... see below
     prolong: nc(4 + 3s)           dot: 16n   waxpby / waxpby_dot: 24n"""
     if kind == "spmv":
-        return m * (v + 4) + 2 * n * s
+        return m * (v + i) + 2 * n * s
     if kind == "symgs":
-        return 2 * (m * (v + 4) + 3 * n * s)
+        return 2 * (m * (v + i) + 3 * n * s)
     if kind == "gs_step":
-        return m * (v + 4) + 3 * n * s
+        return m * (v + i) + 3 * n * s
     if kind == "restrict":
         # injected rows only: their nonzeros (mc = m/8 for the HPCG grid), the
         # fine x once, r at those rows, inj index and the coarse write
-        return mc * (v + 4) + n * s + nc * (4 + 2 * s)
+        return mc * (v + i) + n * s + nc * (4 + 2 * s)
     if kind == "prolong":
         return nc * (4 + 3 * s)
     if kind == "dot":
@@ -83,21 +85,25 @@ def kernel_bytes(kind, n, m, v=8, s=8, nc=0, mc=0):
     raise ValueError(kind)
 
 
-def algorithmic_bytes_per_iteration(nx, ny, nz, levels=4, v_mg=8, s_mg=8, fused=True):
+def algorithmic_bytes_per_iteration(nx, ny, nz, levels=4, v_mg=8, s_mg=8, fused=True, i_mg=None):
     """Bytes of one CG iteration: MG V-cycle + SpMV + 3 dots + 3 WAXPBYs
-    (fused: rz dot 16n, p update 24n, SpMV+dot, update+dot 48n)."""
+    (fused: rz dot 16n, p update 24n, SpMV+dot, update+dot 48n).  i_mg:
+    column-index bytes of the MG matrices (default 4 for fp64 values, 1 for
+    the compact reduced-precision copies)."""
+    if i_mg is None:
+        i_mg = 4 if v_mg == 8 else 1
     shapes = level_shapes(nx, ny, nz, levels)
     n0, m0, _ = shapes[0]
     total = 0.0
     for l, (n, m, d) in enumerate(shapes):
-        total += kernel_bytes("symgs", n, m, v_mg, s_mg)          # pre (or coarsest)
+        total += kernel_bytes("symgs", n, m, v_mg, s_mg, i=i_mg)          # pre (or coarsest)
         if l + 1 < levels:
             nc, mcl, _ = shapes[l + 1]
-            total += kernel_bytes("symgs", n, m, v_mg, s_mg)      # post
+            total += kernel_bytes("symgs", n, m, v_mg, s_mg, i=i_mg)      # post
             if fused:
-                total += kernel_bytes("restrict", n, m, v_mg, s_mg, nc=nc, mc=m // 8)
+                total += kernel_bytes("restrict", n, m, v_mg, s_mg, nc=nc, mc=m // 8, i=i_mg)
             else:
-                total += kernel_bytes("spmv", n, m, v_mg, s_mg) + nc * (4 + 3 * s_mg)
+                total += kernel_bytes("spmv", n, m, v_mg, s_mg, i=i_mg) + nc * (4 + 3 * s_mg)
             total += kernel_bytes("prolong", n, m, v_mg, s_mg, nc=nc)
     if v_mg != 8 or s_mg != 8:
         total += 24 * n0  # casts at the MG boundary
