@@ -44,8 +44,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict
 }
 
 // Ap = A p and the partials of pAp = p.Ap.  One 256-row tile per block.
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const int *__restrict__ cols,
-                                                        const double *__restrict__ vals,
+// CI: column indices from the compact form (the mixed-precision solvers,
+// which already stand outside the reference's rules; values stay fp64).
+template <bool CI>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const ColIdx ci, const double *__restrict__ vals,
                                                         const double *__restrict__ p,
                                                         double *__restrict__ Ap, int64_t nrows,
                                                         int64_t block_rows, double *partials) {
@@ -53,13 +55,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const int *__restrict__ 
   double acc = 0.0;
   if (row < nrows) {
     const size_t b = ell_base(row);
-    double sum = 0.0;
+    const int slice = slice_of<true>(row);
+    auto row_sum = [&](auto cm) {
+      constexpr int CM = decltype(cm)::value;
+      double sum = 0.0;
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-      const int c = ld_col(cols + b + s * kSlice);
-      const double v = ld_val<double>(vals + b + s * kSlice);
-      sum += v * p[c];
-    }
+      for (int s = 0; s < kSlots; ++s) {
+        const int c = col_at<CM, true>(ci, row, slice, b, s);
+        const double v = ld_val<double>(vals + b + s * kSlice);
+        sum += v * p[c];
+      }
+      return sum;
+    };
+    double sum;
+    if constexpr (CI) sum = slice_wide<CI>(ci, slice) ? row_sum(cmode<0>{}) : row_sum(cmode<1>{});
+    else sum = row_sum(cmode<0>{});
     Ap[row] = sum;
     acc = p[row] * sum;
   }
@@ -255,7 +265,8 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
     tm.mark(2);
     k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, k == 1);
     tm.mark(3);
-    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, L0.lay.interleave, c->partials);
+    k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p, Ap,
+                                                       np, L0.lay.interleave, c->partials);
     finish(c, 1, T, F_PAP);
     tm.mark(4);
     k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->scal);
@@ -327,7 +338,12 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
       k_cg_p<float><<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, c->scal, 0);
     restart = false;
     tm.mark(3);
-    k_cg_spmv_dot<<<T, kBlock, 0, c->stream>>>(L0.cols, L0.vals, p, Ap, np, L0.lay.interleave, c->partials);
+    if (L0.cbase)
+      k_cg_spmv_dot<true><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, L0.cbase, L0.cdelta, L0.cwide}, L0.vals, p,
+                                                       Ap, np, L0.lay.interleave, c->partials);
+    else
+      k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p, Ap,
+                                                        np, L0.lay.interleave, c->partials);
     finish(c, 1, T, F_PAP);
     tm.mark(4);
     k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal);

@@ -653,13 +653,15 @@ __global__ void k_sigma(const double *__restrict__ fvals, const uint8_t *__restr
 }
 
 // Compact column indices, one wave per 64-row slice.  For each slot the
-// columns of the rows with a nonzero value are base + lane + d with
-// d in [-128, 127]; a zero-valued slot (neighbour outside the grid) may name
-// any row of the level, as its product is an exact zero: it takes its own row
-// when reachable, else the nearest reachable one.  A slice whose columns do
-// not fit is flagged wide and keeps its explicit indices.
+// columns of the rows with a nonzero value (and of the diagonal, whatever its
+// value) are base + lane + d with d in [-128, 127]; a zero-valued slot
+// (neighbour outside the grid, unused entry) may name any row of the level,
+// as its product is an exact zero: it takes its own row when reachable, else
+// the nearest reachable one.  A slice whose columns do not fit is flagged
+// wide and keeps its explicit indices.
 __global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__ cols,
                                                          const double *__restrict__ vals,
+                                                         const uint8_t *__restrict__ dslot,
                                                          int *__restrict__ cbase, int8_t *__restrict__ cdelta,
                                                          uint8_t *__restrict__ cwide, int64_t nslices, int64_t npad) {
   const int lane = threadIdx.x & 63;
@@ -667,10 +669,11 @@ __global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__
   if (slice >= nslices) return;
   const int64_t row = slice * kSlice + lane;
   const size_t b = (size_t)slice * (kSlots * kSlice) + lane;
+  const int ds = dslot ? dslot[row] : kDiagSlot;
   bool wide = false;
   for (int s = 0; s < kSlots; ++s) {
     const int col = cols[b + s * kSlice];
-    const bool live = vals[b + s * kSlice] != 0.0;
+    const bool live = vals[b + s * kSlice] != 0.0 || s == ds;
     int lo = live ? col - lane : INT_MAX, hi = live ? col - lane : INT_MIN;
     for (int m = 32; m >= 1; m >>= 1) {
       lo = min(lo, __shfl_xor(lo, m, 64));
@@ -705,7 +708,8 @@ static int build_compact_cols(Ctx *c, Level &L) {
   if ((rc = dalloc(c, (void **)&L.cdelta, (size_t)L.lay.npad * kSlots * sizeof(int8_t), false))) return rc;
   if ((rc = dalloc(c, (void **)&L.cwide, (size_t)ns, false))) return rc;
   const int64_t grid = (ns + (kBlock / kSlice) - 1) / (kBlock / kSlice);
-  k_compact_cols<<<grid, kBlock, 0, c->stream>>>(L.cols, L.vals, L.cbase, L.cdelta, L.cwide, ns, L.lay.npad);
+  k_compact_cols<<<grid, kBlock, 0, c->stream>>>(L.cols, L.vals, L.dslot, L.cbase, L.cdelta, L.cwide, ns,
+                                                 L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
