@@ -50,8 +50,8 @@ template <bool CI>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const ColIdx ci, const double *__restrict__ vals,
                                                         const double *__restrict__ p,
                                                         double *__restrict__ Ap, int64_t nrows,
-                                                        int64_t block_rows, double *partials) {
-  const int64_t row = map_row(block_rows, tile_id());
+                                                        double *partials) {
+  const int64_t row = tile_id() * kBlock + threadIdx.x;
   double acc = 0.0;
   if (row < nrows) {
     const size_t b = ell_base(row);
@@ -172,8 +172,8 @@ __global__ __launch_bounds__(kBlock) void k_copy_z(int64_t n, const float *__res
 __global__ __launch_bounds__(kBlock) void k_kappa_grad(const int *__restrict__ cols, const int *__restrict__ iperm,
                                                        const double *__restrict__ lam,
                                                        const double *__restrict__ x, double *__restrict__ g,
-                                                       int64_t npad, int64_t block_rows) {
-  const int64_t p = map_row(block_rows, tile_id());
+                                                       int64_t npad) {
+  const int64_t p = tile_id() * kBlock + threadIdx.x;
   if (p >= npad) return;
   if (iperm[p] < 0) {
     g[p] = 0.0;
@@ -266,7 +266,7 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
     k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, k == 1);
     tm.mark(3);
     k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p, Ap,
-                                                       np, L0.lay.interleave, c->partials);
+                                                       np, c->partials);
     finish(c, 1, T, F_PAP);
     tm.mark(4);
     k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->scal);
@@ -340,10 +340,10 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
     tm.mark(3);
     if (L0.cbase)
       k_cg_spmv_dot<true><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, L0.cbase, L0.cdelta, L0.cwide}, L0.vals, p,
-                                                       Ap, np, L0.lay.interleave, c->partials);
+                                                       Ap, np, c->partials);
     else
       k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p, Ap,
-                                                        np, L0.lay.interleave, c->partials);
+                                                        np, c->partials);
     finish(c, 1, T, F_PAP);
     tm.mark(4);
     k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal);
@@ -400,8 +400,7 @@ int hpcgx_kappa_grad(hpcgx_ctx *c, const double *lam, const double *x, double *g
   if (rc) return rc;
   const Level &L = c->levels[0];
   if (L.lay.general) return set_err(c, HPCGX_EINVAL, "kappa_grad: the kappa operator is a 27-point stencil level");
-  k_kappa_grad<<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.iperm, lam, x, g, L.lay.npad,
-                                                            L.lay.interleave);
+  k_kappa_grad<<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.iperm, lam, x, g, L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }

@@ -66,8 +66,7 @@ struct Layout {
   int cls[kColours] = {};                // parity class of block k
   int pos[kColours] = {};                // block of parity class c
   int inj = 0;                           // block of class 0 (the injected rows)
-  int64_t interleave = 0;                // padded rows per block if all 8 are equal and
-                                         // a multiple of 256 (map_row), else 0
+  int strip = 0;                         // rows order of a block (block_index), 0: natural
 };
 
 Layout make_layout(int nx, int ny, int nz, const int *order);
@@ -82,11 +81,41 @@ struct LayoutDev {  // by-value kernel argument
   int nb[kColours][3];
   int cls[kColours];
   int pos[kColours];
+  int strip;
 };
+
+// Row order inside a block of extents (NX, NY, NZ).  strip == 0: natural
+// order of (X, Y, Z).  strip == H: the block is cut into NY / H strips of H
+// rows of Y, taken one after another, each in natural order of (X, y, Z).
+// The 8 XCDs each sweep a contiguous eighth of a block; with 8 strips each
+// XCD walks one strip through all Z, so the neighbour entries of one plane
+// are re-read while the next plane is in flight and stay in that XCD's L2
+// (in natural order a whole plane of a large grid lies between the two
+// reads of an entry).
+__host__ __device__ inline int64_t block_index(const LayoutDev &L, int k, int X, int Y, int Z) {
+  const int64_t NX = L.nb[k][0], NY = L.nb[k][1], NZ = L.nb[k][2];
+  if (!L.strip) return ((int64_t)Z * NY + Y) * NX + X;
+  const int64_t H = L.strip;
+  return (Y / H) * (H * NX * NZ) + ((int64_t)Z * H + Y % H) * NX + X;
+}
+
+__host__ __device__ inline void block_coords(const LayoutDev &L, int k, int64_t idx, int &X, int &Y, int &Z) {
+  const int64_t NX = L.nb[k][0], NY = L.nb[k][1];
+  X = (int)(idx % NX);
+  if (!L.strip) {
+    Y = (int)((idx / NX) % NY);
+    Z = (int)(idx / (NX * NY));
+    return;
+  }
+  const int64_t H = L.strip, NZ = L.nb[k][2];
+  const int64_t s = idx / (H * NX * NZ), rem = idx % (H * NX * NZ);
+  Z = (int)(rem / (H * NX));
+  Y = (int)(s * H + (rem / NX) % H);
+}
 
 __host__ __device__ inline int64_t perm_of(const LayoutDev &L, int ix, int iy, int iz) {
   const int k = L.pos[parity_class(ix, iy, iz)];
-  return L.start[k] + ((int64_t)(iz >> 1) * L.nb[k][1] + (iy >> 1)) * L.nb[k][0] + (ix >> 1);
+  return L.start[k] + block_index(L, k, ix >> 1, iy >> 1, iz >> 1);
 }
 
 struct Level {

@@ -30,25 +30,6 @@ __device__ __forceinline__ int64_t tile_id() {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-// Row of thread threadIdx.x in the tile-th 256-row tile of a whole-level
-// kernel.  When the 8 colour blocks have equal padded sizes that are a
-// multiple of 256 (even grids), tiles interleave the blocks (tile t -> block
-// t % 8, tile t / 8 inside it): concurrently running tiles then cover all 8
-// classes of one region of the grid, so a gathered vector entry is fetched
-// once and reused by its neighbours of every colour, instead of once per
-// colour block.  Otherwise rows run in storage order.
-__device__ __forceinline__ int64_t map_row(int64_t block_rows, int64_t tile) {
-  if (block_rows == 0) return tile * kBlock + threadIdx.x;
-  return (tile & 7) * block_rows + (tile >> 3) * kBlock + threadIdx.x;
-}
-
-// Minimum resident blocks per CU the sweep kernel is compiled for (bounds
-// its registers: 256-thread blocks, 4 SIMDs, so 8 blocks = 8 waves per SIMD).
-#ifndef HPCGX_GS_MIN_BLOCKS
-#define HPCGX_GS_MIN_BLOCKS 1
-#endif
-constexpr int kGsMinBlocks = HPCGX_GS_MIN_BLOCKS;
-
 // Matrix loads.  NT (non-temporal): the fine level's matrix is streamed once
 // per pass and is far larger than any cache, so it is kept out of the way of
 // the vectors; coarse-level matrices fit in the 256 MB Infinity Cache and
@@ -238,8 +219,8 @@ template <class VT, class XT, class ACC>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ cols,
                                                  const VT *__restrict__ vals,
                                                  const XT *__restrict__ x, XT *__restrict__ y,
-                                                 int64_t nrows, int64_t block_rows) {
-  const int64_t row = map_row(block_rows, tile_id());
+                                                 int64_t nrows) {
+  const int64_t row = tile_id() * kBlock + threadIdx.x;
   if (row >= nrows) return;
   const size_t b = ell_base(row);
   ACC sum = 0;
@@ -269,13 +250,14 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
 // sum = r_i - sum_j a_ij x_j (diagonal included) + x_i a_ii ; x_i = sum / a_ii.
 // ZK >= 0: the step of block ZK in the first forward sweep from x = 0 -- only
 // neighbours in blocks swept before ZK can be nonzero; the other terms are
-// exact zeros and are not read, and neither is x_i.  The skipped set is a
-// compile-time constant, so the remaining loads issue back to back.
+// exact zeros and are not read, and neither is x_i (x may hold anything on
+// entry).  The skipped set is a compile-time constant, so the remaining loads
+// issue back to back.
 // FINE: the finest level (streamed matrix loads; its own instantiation, so
 // profiles show the dominant launches separately from the coarse levels').
 // GEN: general level, the diagonal's slot is dslot[row] (stencil: slot 13).
 template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false>
-__global__ __launch_bounds__(kBlock, kGsMinBlocks) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
+__global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
                                                const XT *__restrict__ r, XT *__restrict__ x,
                                                int64_t row0, int64_t nrows,
                                                const uint8_t *__restrict__ dslot) {
@@ -297,7 +279,12 @@ __global__ __launch_bounds__(kBlock, kGsMinBlocks) void k_gs(const ColIdx ci, co
       if (!((mask >> s) & 1u)) continue;
       const int c = col_at<CM, FINE>(ci, row, slice, b, s);
       const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
-      sum -= v * (ACC)x[c];
+      const ACC xc = (ACC)x[c];
+      // From x = 0, a zero entry (a neighbour outside the grid) may point at
+      // a row not written yet in this sweep, whose stale value is not
+      // trusted: it contributes 0 * 0, as the reference's zeroed x gives.
+      if constexpr (ZK >= 0) sum -= v * (v != (ACC)0 ? xc : (ACC)0);
+      else sum -= v * xc;
     }
     return sum;
   };

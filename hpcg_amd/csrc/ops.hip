@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_to_double(const float *__restrict__ 
 int spmv_fp64(Ctx *c, int level, const double *x, double *y) {
   const Level &L = c->levels[level];
   k_spmv<double, double, double><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, x, y,
-                                                                              L.lay.npad, L.lay.interleave);
+                                                                              L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }

@@ -82,10 +82,13 @@ Layout make_layout(int nx, int ny, int nz, const int *order) {
     L.start[k + 1] = L.start[k] + padded;
   }
   L.npad = L.start[kColours];
-  const int64_t b0 = L.start[1] - L.start[0];
-  bool equal = b0 % kBlock == 0;
-  for (int k = 1; k < kColours; ++k) equal = equal && (L.start[k + 1] - L.start[k] == b0);
-  L.interleave = equal ? b0 : 0;
+  // 8 strips of Y when every block has the same Y extent and a plane of a
+  // block is large (>= 128 rows of Y: at 192^3 and below the vectors stay in
+  // the 256 MB last-level cache anyway)
+  const int nby = L.nb[0][1];
+  bool strips = ny % 2 == 0 && nby >= 128 && nby % 8 == 0;
+  for (int k = 1; k < kColours; ++k) strips = strips && L.nb[k][1] == nby;
+  L.strip = strips ? nby / 8 : 0;
   return L;
 }
 
@@ -94,6 +97,7 @@ static LayoutDev to_dev(const Layout &L) {
   d.nx = L.nx;
   d.ny = L.ny;
   d.nz = L.nz;
+  d.strip = L.strip;
   for (int k = 0; k <= kColours; ++k) d.start[k] = L.start[k];
   for (int k = 0; k < kColours; ++k) {
     for (int a = 0; a < 3; ++a) d.nb[k][a] = L.nb[k][a];
@@ -222,9 +226,9 @@ __global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, 
     return;
   }
   const int c = L.cls[k];
-  const int ix = 2 * (int)(idx % nbx) + (c & 1);
-  const int iy = 2 * (int)((idx / nbx) % nby) + ((c >> 1) & 1);
-  const int iz = 2 * (int)(idx / ((int64_t)nbx * nby)) + ((c >> 2) & 1);
+  int X, Y, Z;
+  block_coords(L, k, idx, X, Y, Z);
+  const int ix = 2 * X + (c & 1), iy = 2 * Y + ((c >> 1) & 1), iz = 2 * Z + ((c >> 2) & 1);
   const int64_t i = ((int64_t)iz * L.ny + iy) * L.nx + ix;
   iperm[p] = (int)i;
   perm[i] = (int)p;
@@ -263,10 +267,13 @@ __global__ void k_inject_kappa(const double *__restrict__ kf, double *__restrict
 }
 
 // inj for generated levels: the t-th injected fine row is coarse natural row t
-__global__ void k_inj_generated(LayoutDev Lc, int *__restrict__ inj, int64_t nc) {
+__global__ void k_inj_generated(LayoutDev Lf, LayoutDev Lc, int *__restrict__ inj, int64_t nc) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= nc) return;
-  const int a = (int)(t % Lc.nx), b = (int)((t / Lc.nx) % Lc.ny), cc = (int)(t / ((int64_t)Lc.nx * Lc.ny));
+  // the t-th row of the fine class-0 block is the fine point (2a, 2b, 2c):
+  // coarse point (a, b, c) (GenerateCoarseProblem.cpp:71-82)
+  int a, b, cc;
+  block_coords(Lf, Lf.pos[0], t, a, b, cc);
   inj[t] = (int)perm_of(Lc, a, b, cc);
 }
 
@@ -281,16 +288,16 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kB
 template <class RowAcc>
 static bool pack_stencil(Level &L, RowAcc rows, std::vector<int> &hc, std::vector<double> &hv) {
   const Layout &lay = L.lay;
-  const int nx = lay.nx, ny = lay.ny, nz = lay.nz;
+  const int nx = lay.nx, ny = lay.ny;
   const int64_t n = lay.n, npad = lay.npad;
   L.perm_h.assign(n, 0);
   L.iperm_h.assign(npad, -1);
   for (int k = 0; k < kColours; ++k) {
     const int cl = lay.cls[k];
     for (int64_t idx = 0; idx < lay.count[k]; ++idx) {
-      const int ix = 2 * (int)(idx % lay.nb[k][0]) + (cl & 1);
-      const int iy = 2 * (int)((idx / lay.nb[k][0]) % lay.nb[k][1]) + ((cl >> 1) & 1);
-      const int iz = 2 * (int)(idx / ((int64_t)lay.nb[k][0] * lay.nb[k][1])) + ((cl >> 2) & 1);
+      int X, Y, Z;
+      block_coords(L.ldev, k, idx, X, Y, Z);
+      const int ix = 2 * X + (cl & 1), iy = 2 * Y + ((cl >> 1) & 1), iz = 2 * Z + ((cl >> 2) & 1);
       const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
       L.perm_h[i] = (int)(lay.start[k] + idx);
       L.iperm_h[lay.start[k] + idx] = (int)i;
@@ -403,7 +410,7 @@ static int pack_general(Ctx *c, Level &L, RowAcc rows, std::vector<int> &hc, std
   Layout &lay = L.lay;
   lay.general = true;
   lay.ncol = ncol;
-  lay.interleave = 0;
+  lay.strip = 0;
   lay.inj = 0;
   for (int k = 0; k < kMaxColours; ++k) lay.count[k] = 0;
   for (int64_t i = 0; i < n; ++i) lay.count[colour[i]]++;
@@ -943,7 +950,7 @@ int hpcgx_problem_generate(hpcgx_ctx *c, int nx, int ny, int nz, int levels, con
     const Level &Cc = c->levels[l + 1];
     if ((rc = dalloc(c, (void **)&F.inj, Cc.lay.n * sizeof(int), false))) return rc;
     F.ninj = Cc.lay.n;
-    k_inj_generated<<<nblk(Cc.lay.n), kBlock, 0, c->stream>>>(Cc.ldev, F.inj, Cc.lay.n);
+    k_inj_generated<<<nblk(Cc.lay.n), kBlock, 0, c->stream>>>(F.ldev, Cc.ldev, F.inj, Cc.lay.n);
     HPCGX_CHECK_LAUNCH(c);
   }
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));

@@ -170,14 +170,17 @@ def test_mixed_cg_converges(hip, variant):
 
 def _kappa(n, seed=7):
     g = rng(seed)
-    return np.exp(0.5 * g.standard_normal(n ** 3))
+    return np.exp(0.5 * g.standard_normal(n))
 
 
-def test_kappa_operator_and_gradient_bitexact(hip):
-    n = 16
-    kappa = _kappa(n)
-    ctx = device_problem(hip, n, kappa=kappa)
-    P = po.Problem(n, n, n, kappa=kappa)
+@pytest.mark.parametrize("grid", [(16, 16, 16), (16, 256, 8)])
+def test_kappa_operator_and_gradient_bitexact(hip, grid):
+    """(16, 256, 8) also covers the Y-strip row order of blocks with >= 128
+    rows of Y."""
+    kappa = _kappa(int(np.prod(grid)))
+    ctx = hip.Context(0)
+    ctx.generate(*grid, levels=4, kappa=kappa)
+    P = po.Problem(*grid, kappa=kappa)
     for l in range(4):
         onr, ocols, ovals, _ = P.level(l).arrays()
         nr, cols, vals = ctx.export_level(l)
@@ -190,6 +193,10 @@ def test_kappa_operator_and_gradient_bitexact(hip):
     dl, dx, dg = ctx.vec(0, lam), ctx.vec(0, x), ctx.vec(0)
     ctx.kappa_grad(dl, dx, dg)
     assert np.array_equal(dg.download(), P.kappa_grad(lam, x))
+    r = rng(10).random(P.n)
+    dr, dz = ctx.vec(0, r), ctx.vec(0)
+    ctx.mg(0, dr, dz)
+    assert np.array_equal(dz.download(), P.mg(r, mode=2))
 
 
 def test_kappa_one_is_hpcg_matrix(hip):

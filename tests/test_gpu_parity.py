@@ -13,7 +13,9 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(16, 16, 16), (24, 24, 24), (32, 16, 24)]
+# (16, 256, 8) and (32, 256, 16) have blocks with >= 128 rows of Y: their
+# finest rows are stored in Y strips (block_index), the others in natural order.
+GRIDS = [(16, 16, 16), (24, 24, 24), (32, 16, 24), (16, 256, 8), (32, 256, 16)]
 
 
 @pytest.fixture(scope="module")
@@ -221,3 +223,46 @@ def test_other_colour_order_matches_oracle(hip):
     assert np.array_equal(dx.download(), P.symgs(r, x0, mode=2))
     ctx.mg(0, dr, dz)
     assert np.array_equal(dz.download(), P.mg(r, mode=2))
+
+
+@pytest.mark.parametrize("grid,levels", [((15, 9, 7), 1), ((1, 1, 1), 1), ((2, 3, 1), 1), ((30, 18, 14), 2)])
+def test_ragged_grids(hip, grid, levels):
+    """Odd extents give colour blocks of different sizes (and empty ones on
+    1-wide grids); the single-row grid is the smallest problem.  SpMV, the
+    symmetric Gauss-Seidel and the V-cycle stay bit-exact, and the CG
+    history follows the oracle's."""
+    ctx = hip.Context(0)
+    ctx.generate(*grid, levels=levels)
+    P = po.Problem(*grid, levels=levels)
+    x = rng(100).random(P.n)
+    dx, dy = ctx.vec(0, x), ctx.vec(0)
+    ctx.spmv(0, dx, dy)
+    assert np.array_equal(dy.download(), P.spmv(x))
+    r, x0 = rng(101).random(P.n), rng(102).random(P.n)
+    dr, dx0, dz = ctx.vec(0, r), ctx.vec(0, x0), ctx.vec(0)
+    ctx.symgs(0, dr, dx0)
+    assert np.array_equal(dx0.download(), P.symgs(r, x0, mode=2))
+    ctx.mg(0, dr, dz)
+    assert np.array_equal(dz.download(), P.mg(r, mode=2))
+    b = P.rhs()[0]
+    db, dxx = ctx.vec(0, b), ctx.vec(0)
+    got = ctx.cg(db, dxx, max_iter=10, tol=0.0, history=True)
+    want = P.cg(b, max_iter=10, mode=2, history=True)
+    assert got["niters"] == want["niters"]
+    live = want["hist"] / want["normr0"] > 1e-10
+    assert np.allclose(got["hist"][live] / got["normr0"], want["hist"][live] / want["normr0"], rtol=1e-8)
+    ctx.close()
+
+
+@pytest.mark.parametrize("g", [(16, 16, 16), (16, 256, 8)])
+def test_mg_ignores_output_contents(hip, g):
+    """ComputeMG_ref.cpp:43 zeroes x before the V-cycle: whatever the output
+    vector holds on entry (here NaN) must not reach the result."""
+    ctx = hip.Context(0)
+    ctx.generate(*g)
+    P = po.Problem(*g)
+    r = rng(110).random(P.n)
+    dr, dz = ctx.vec(0, r), ctx.vec(0, np.full(P.n, np.nan))
+    ctx.mg(0, dr, dz)
+    assert np.array_equal(dz.download(), P.mg(r, mode=2))
+    ctx.close()
