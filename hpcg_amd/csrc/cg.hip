@@ -245,6 +245,7 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
   double *r = c->cg_r, *z = precond ? c->cg_z : c->cg_r, *p = c->cg_p, *Ap = c->cg_Ap;
   int rc;
   PhaseTimer tm{c, times != nullptr};
+  const bool fuse_rz = precond && c->levels.size() > 1 && !L0.lay.general;
   // preamble: Ap = A x ; r = b - Ap ; normr = ||r||  (CG.cpp:86-90)
   if ((rc = spmv_fp64(c, 0, x, Ap))) return rc;
   k_cg_residual<false><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, nullptr, c->partials);
@@ -258,10 +259,17 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
   *niters = 0;
   for (int k = 1; k <= max_iter && normr / normr0 > tol; ++k) {
     tm.mark(0);
-    if (precond && (rc = mg_fp64(c, 0, r, z))) return rc;
-    tm.mark(1);
-    k_cg_rz<double><<<G, kBlock, 0, c->stream>>>(np, r, z, c->partials);
-    finish(c, 1, G, k == 1 ? F_RZ_FIRST : F_RZ);
+    if (fuse_rz) {  // r.z partials from the V-cycle's post-smoother
+      int64_t nrz = 0;
+      if ((rc = mg_fp64(c, 0, r, z, c->partials, &nrz))) return rc;
+      tm.mark(1);
+      finish(c, 1, nrz, k == 1 ? F_RZ_FIRST : F_RZ);
+    } else {
+      if (precond && (rc = mg_fp64(c, 0, r, z))) return rc;
+      tm.mark(1);
+      k_cg_rz<double><<<G, kBlock, 0, c->stream>>>(np, r, z, c->partials);
+      finish(c, 1, G, k == 1 ? F_RZ_FIRST : F_RZ);
+    }
     tm.mark(2);
     k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, k == 1);
     tm.mark(3);

@@ -176,6 +176,10 @@ struct Ctx {
   int64_t cg_len = 0, cg32_len = 0;  // elements the solver buffers hold
   double *stage = nullptr;     // natural-order staging for upload/download
   int64_t stage_len = 0;
+  // double-buffered row staging of level uploads (pinned host, device)
+  uint8_t *up_host[2] = {}, *up_dev[2] = {};
+  hipEvent_t up_ev[2] = {};
+  int *up_err = nullptr;       // [64] per-lane error flags of the device packing
   std::vector<void *> user_vecs;
   int64_t bytes = 0;
   bool ready = false;
@@ -200,7 +204,8 @@ int ensure_ready(Ctx *c);
 int ensure_stage(Ctx *c, int64_t n);
 
 // launch helpers (implemented in ops.hip)
-int mg_fp64(Ctx *c, int level, const double *r, double *x);
+// rz: if given, the partials of r.x (result) are left there, *nrz of them
+int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz = nullptr, int64_t *nrz = nullptr);
 int mg_fp32(Ctx *c, int level, const float *r, float *x);
 int mg_fp16(Ctx *c, const float *r, float *x);
 int spmv_fp64(Ctx *c, int level, const double *x, double *y);

@@ -256,16 +256,12 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
 // FINE: the finest level (streamed matrix loads; its own instantiation, so
 // profiles show the dominant launches separately from the coarse levels').
 // GEN: general level, the diagonal's slot is dslot[row] (stencil: slot 13).
-template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false>
-__global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
-                                               const XT *__restrict__ r, XT *__restrict__ x,
-                                               int64_t row0, int64_t nrows,
-                                               const uint8_t *__restrict__ dslot) {
-  static_assert(!GEN || ZK < 0, "general levels sweep full rows");
+// gs_row is the update of one row; it returns r_i * x_i (new) for the fused r.z.
+template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI, bool GEN>
+__device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict__ vals,
+                                         const XT *__restrict__ r, XT *__restrict__ x, int64_t row,
+                                         const uint8_t *__restrict__ dslot) {
   constexpr uint32_t mask = ZK >= 0 ? swept_before_mask(ZK) : 0x7ffffffu;
-  const int64_t t = tile_id() * kBlock + threadIdx.x;
-  if (t >= nrows) return;
-  const int64_t row = row0 + t;
   const size_t b = ell_base(row);
   const int slice = slice_of<true>(row);
   int ds = kDiagSlot;
@@ -296,7 +292,33 @@ __global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__rest
     sum = row_sum(cmode<0>{});
   }
   if (ZK < 0) sum += (ACC)x[row] * d;
-  x[row] = (XT)(sum / d);
+  const XT xn = (XT)(sum / d);
+  x[row] = xn;
+  return (double)r[row] * (double)xn;
+}
+
+// RZ: this step writes the final value of its rows in the V-cycle (the
+// post-smoother's last forward block and its backward steps); the block
+// also leaves its partial sum of r_i x_i in rz[blockIdx.x], so the solver's
+// r.z needs no pass of its own.
+template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false, bool RZ = false>
+__global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
+                                               const XT *__restrict__ r, XT *__restrict__ x,
+                                               int64_t row0, int64_t nrows,
+                                               const uint8_t *__restrict__ dslot, double *rz = nullptr) {
+  static_assert(!GEN || ZK < 0, "general levels sweep full rows");
+  static_assert(!RZ || ZK < 0, "r.z is taken from the post-smoother");
+  const int64_t t = tile_id() * kBlock + threadIdx.x;
+  if constexpr (RZ) {
+    // every thread reaches the block reduction; rows past the block add 0
+    double acc = 0.0;
+    if (t < nrows) acc = gs_row<VT, XT, ACC, ZK, FINE, CI, GEN>(ci, vals, r, x, row0 + t, dslot);
+    const double v[1] = {acc};
+    block_partials<1>(v, rz);
+    return;
+  }
+  if (t >= nrows) return;
+  gs_row<VT, XT, ACC, ZK, FINE, CI, GEN>(ci, vals, r, x, row0 + t, dslot);
 }
 
 // Residual at the injected rows and restriction, fused (ComputeMG_ref.cpp:49-51 +

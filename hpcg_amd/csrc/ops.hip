@@ -24,13 +24,21 @@ static ColIdx col_idx(const Level &L) {
   return ColIdx{L.cols, nullptr, nullptr, nullptr};
 }
 
+// rz (fp64 stencil levels only): the step also leaves the partials of r.x
+// over its rows there, tiles(rows) of them.
 template <class VT, class XT, class ACC, int ZK, bool FINE>
-static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, int k) {
+static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, int k, double *rz = nullptr) {
   const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
   if (nr <= 0) return;
   const ColIdx ci = col_idx<VT>(L);
   const unsigned g = tiles(nr);
   hipStream_t st = c->stream;
+  if constexpr (ZK < 0 && sizeof(VT) == 8 && sizeof(XT) == 8) {
+    if (rz) {
+      k_gs<VT, XT, ACC, -1, FINE, false, false, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr, rz);
+      return;
+    }
+  }
   if constexpr (ZK < 0) {
     if (L.dslot) {
       if (ci.cbase)
@@ -65,8 +73,11 @@ static bool default_order(const Level &L) {
 // used, so it is elided.  zero_start: x is known to be zero on entry (the
 // V-cycle's first sweep); in the default stencil order x is then not read
 // before it is written, otherwise it is zero-filled first.
+// rz: the steps that leave their rows final (forward step of the last block,
+// then the backward steps) write r.x partials at rz, *nrz of them in all.
 template <class VT, class XT, class ACC, bool FINE>
-static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start) {
+static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start, double *rz = nullptr,
+                  int64_t *nrz = nullptr) {
   const VT *v = vals_of<VT>(L);
   if (zero_start && !default_order(L)) {  // other orders: zero x, then full steps
     k_zero<XT><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(x, L.lay.npad);
@@ -82,24 +93,37 @@ static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start) {
     gs_block<VT, XT, ACC, 6, FINE>(c, L, v, r, x, 6);
     gs_block<VT, XT, ACC, 7, FINE>(c, L, v, r, x, 7);
   } else {
-    for (int k = 0; k < L.lay.ncol; ++k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
+    for (int k = 0; k + 1 < L.lay.ncol; ++k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
+    const int last = L.lay.ncol - 1;
+    gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, last, rz);
+    if (rz) rz += tiles(L.lay.start[last + 1] - L.lay.start[last]);
   }
-  for (int k = L.lay.ncol - 2; k >= 0; --k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
+  for (int k = L.lay.ncol - 2; k >= 0; --k) {
+    gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k, rz);
+    if (rz) rz += tiles(L.lay.start[k + 1] - L.lay.start[k]);
+  }
+  if (nrz) {
+    *nrz = 0;
+    for (int k = 0; k < L.lay.ncol; ++k) *nrz += tiles(L.lay.start[k + 1] - L.lay.start[k]);
+  }
 }
 
 template <class VT, class XT, class ACC>
-static int symgs_launch(Ctx *c, int l, const XT *r, XT *x, bool zero_start) {
+static int symgs_launch(Ctx *c, int l, const XT *r, XT *x, bool zero_start, double *rz = nullptr,
+                        int64_t *nrz = nullptr) {
   const Level &L = c->levels[l];
-  if (l == 0) sweep<VT, XT, ACC, true>(c, L, r, x, zero_start);
-  else sweep<VT, XT, ACC, false>(c, L, r, x, zero_start);
+  if (l == 0) sweep<VT, XT, ACC, true>(c, L, r, x, zero_start, rz, nrz);
+  else sweep<VT, XT, ACC, false>(c, L, r, x, zero_start, rz, nrz);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
 
 // V-cycle of ComputeMG_ref.cpp:39-63: x = 0; pre-smooth; residual + restrict;
 // recurse; prolong; post-smooth.  The coarsest level gets one smoothing step.
+// rz / nrz: r.x partials of the result from the post-smoother (see sweep);
+// fp64 stencil levels with a coarse level only.
 template <class VT, class XT, class ACC, bool SCALE>
-static int vcycle(Ctx *c, int l, const XT *r, XT *x) {
+static int vcycle(Ctx *c, int l, const XT *r, XT *x, double *rz = nullptr, int64_t *nrz = nullptr) {
   const Level &L = c->levels[l];
   int rc = symgs_launch<VT, XT, ACC>(c, l, r, x, true);
   if (rc || l + 1 >= (int)c->levels.size()) return rc;
@@ -124,7 +148,7 @@ static int vcycle(Ctx *c, int l, const XT *r, XT *x) {
   if ((rc = vcycle<VT, XT, ACC, SCALE>(c, l + 1, rc_v, xc_v))) return rc;
   k_prolong<XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc_v, L.inj, L.sigma, r0, nc, L.frow);
   HPCGX_CHECK_LAUNCH(c);
-  return symgs_launch<VT, XT, ACC>(c, l, r, x, false);
+  return symgs_launch<VT, XT, ACC>(c, l, r, x, false, rz, nrz);
 }
 
 // vector kernels
@@ -182,8 +206,10 @@ __global__ __launch_bounds__(kBlock) void k_waxpby_dot(int64_t n, double alpha,
 }
 
 
-int mg_fp64(Ctx *c, int level, const double *r, double *x) {
-  return vcycle<double, double, double, false>(c, level, r, x);
+int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz, int64_t *nrz) {
+  if (rz && (level + 1 >= (int)c->levels.size() || c->levels[level].lay.general))
+    return set_err(c, HPCGX_EINVAL, "fused r.z needs a stencil level with a coarse level");
+  return vcycle<double, double, double, false>(c, level, r, x, rz, nrz);
 }
 
 int mg_fp32(Ctx *c, int level, const float *r, float *x) {

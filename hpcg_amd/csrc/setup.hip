@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_fill(double *__restrict__ v, const i
 __global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, int *__restrict__ cols,
                                                      double *__restrict__ vals,
                                                      int *__restrict__ iperm, int *__restrict__ perm,
-                                                     const double *__restrict__ kappa) {
+                                                     const double *__restrict__ kappa, bool values) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= npad) return;
   int k = 0;
@@ -244,6 +244,10 @@ __global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, 
       continue;
     }
     cols[b + s * kSlice] = (int)perm_of(L, jx, jy, jz);
+    if (!values) {  // structure only: the uploaded values come next
+      vals[b + s * kSlice] = 0.0;
+      continue;
+    }
     if (s == kDiagSlot) continue;
     ++nn;
     if (kappa) {
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, 
       vals[b + s * kSlice] = -1.0;
     }
   }
-  vals[b + kDiagSlot * kSlice] = kappa ? dsum + (double)(26 - nn) * ki : 26.0;
+  if (values) vals[b + kDiagSlot * kSlice] = kappa ? dsum + (double)(26 - nn) * ki : 26.0;
 }
 
 // coarse kappa by injection (GenerateCoarseProblem.cpp:71-82 applied to kappa)
@@ -282,65 +286,140 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kB
 // ---------------------------------------------------------------------------
 // upload from the reference's structures
 
-// Stencil layout: every row is a subset of the 27-point stencil of its
-// (nx, ny, nz) grid, ascending columns, with a diagonal.  Returns false (and
-// leaves the level for the general layout) when the rows are not.
-template <class RowAcc>
-static bool pack_stencil(Level &L, RowAcc rows, std::vector<int> &hc, std::vector<double> &hv) {
-  const Layout &lay = L.lay;
-  const int nx = lay.nx, ny = lay.ny;
-  const int64_t n = lay.n, npad = lay.npad;
-  L.perm_h.assign(n, 0);
-  L.iperm_h.assign(npad, -1);
-  for (int k = 0; k < kColours; ++k) {
-    const int cl = lay.cls[k];
-    for (int64_t idx = 0; idx < lay.count[k]; ++idx) {
-      int X, Y, Z;
-      block_coords(L.ldev, k, idx, X, Y, Z);
-      const int ix = 2 * X + (cl & 1), iy = 2 * Y + ((cl >> 1) & 1), iz = 2 * Z + ((cl >> 2) & 1);
-      const int64_t i = ((int64_t)iz * ny + iy) * nx + ix;
-      L.perm_h[i] = (int)(lay.start[k] + idx);
-      L.iperm_h[lay.start[k] + idx] = (int)i;
-    }
+// ---------------------------------------------------------------------------
+// device-side packing of uploaded stencil rows (OptimizeProblem's heavy part)
+//
+// The host only gathers the rows, in natural order, into pinned chunks of
+// kUpRows rows ([rows][27] columns and values plus the row lengths); the
+// copies run while the next chunk is gathered, and one kernel per chunk
+// checks each row against the stencil of the grid and scatters its values
+// into the slots that k_generate (structure only) laid out.
+
+constexpr int64_t kUpRows = 1 << 18;
+constexpr size_t kUpBytes = (size_t)kUpRows * (1 + kSlots * (sizeof(int) + sizeof(double)));
+
+struct UpChunk {
+  uint8_t *nr;
+  int *cols;
+  double *vals;
+};
+static UpChunk up_view(uint8_t *base) {
+  UpChunk u;
+  u.vals = reinterpret_cast<double *>(base);
+  u.cols = reinterpret_cast<int *>(base + (size_t)kUpRows * kSlots * sizeof(double));
+  u.nr = base + (size_t)kUpRows * kSlots * (sizeof(double) + sizeof(int));
+  return u;
+}
+
+// Row i0 + t of the chunk: natural row i with coordinates (ix, iy, iz).  An
+// entry must be an in-grid neighbour (offset in {-1,0,1}^3), entries must
+// ascend in stencil order, and the diagonal must be present; a row that is
+// not flags err[lane] (the host then packs the level in the general layout).
+__global__ __launch_bounds__(kBlock) void k_pack_rows(LayoutDev L, int64_t n, const uint8_t *__restrict__ nr,
+                                                      const int *__restrict__ scols,
+                                                      const double *__restrict__ svals, int64_t i0,
+                                                      int64_t rows, const int *__restrict__ perm,
+                                                      double *__restrict__ vals, int *__restrict__ err) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= rows) return;
+  const int64_t i = i0 + t;
+  const int ix = (int)(i % L.nx), iy = (int)((i / L.nx) % L.ny), iz = (int)(i / ((int64_t)L.nx * L.ny));
+  const size_t b = ell_base(perm[i]);
+  const int cnt = nr[t];
+  bool ok = cnt >= 1 && cnt <= kSlots, diag = false;
+  int last = -1;
+  for (int j = 0; ok && j < cnt; ++j) {
+    const int col = scols[t * kSlots + j];
+    if (col < 0 || col >= n) { ok = false; break; }
+    const int dx = (int)(col % L.nx) - ix, dy = (int)((col / L.nx) % L.ny) - iy,
+              dz = (int)(col / ((int64_t)L.nx * L.ny)) - iz;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) { ok = false; break; }
+    const int s = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+    if (s <= last) { ok = false; break; }
+    last = s;
+    diag = diag || s == kDiagSlot;
+    vals[b + s * kSlice] = svals[t * kSlots + j];
   }
-  const size_t m = (size_t)npad * kSlots;
-  hc.assign(m, 0);
-  hv.assign(m, 0.0);
-  std::atomic<int> bad(0);
+  if (!ok || !diag) err[threadIdx.x & 63] = 1;
+}
+
+static int ensure_upload_staging(Ctx *c) {
+  for (int k = 0; k < 2; ++k) {
+    if (!c->up_host[k] && hipHostMalloc((void **)&c->up_host[k], kUpBytes, hipHostMallocDefault) != hipSuccess) {
+      c->up_host[k] = nullptr;
+      return set_err(c, HPCGX_ENOMEM, "pinned upload staging");
+    }
+    int rc;
+    if (!c->up_dev[k] && (rc = dalloc(c, (void **)&c->up_dev[k], kUpBytes, false))) return rc;
+    if (!c->up_ev[k]) HPCGX_HIP(c, hipEventCreateWithFlags(&c->up_ev[k], hipEventDisableTiming));
+  }
+  if (!c->up_err) {
+    int rc = dalloc(c, (void **)&c->up_err, 64 * sizeof(int), false);
+    if (rc) return rc;
+  }
+  return HPCGX_OK;
+}
+
+// Packs a stencil level on the device.  *stencil is false when some row is
+// not the stencil of the grid (the level's device arrays are then released).
+template <class RowAcc>
+static int pack_stencil_device(Ctx *c, Level &L, RowAcc rows, bool *stencil) {
+  const Layout &lay = L.lay;
+  const int64_t n = lay.n, npad = lay.npad;
+  int rc;
+  *stencil = false;
+  if ((rc = ensure_upload_staging(c)) || (rc = alloc_level(c, L))) return rc;
+  k_generate<<<nblk(npad), kBlock, 0, c->stream>>>(L.ldev, npad, L.cols, L.vals, L.iperm, L.perm, nullptr, false);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipMemsetAsync(c->up_err, 0, 64 * sizeof(int), c->stream));
   std::atomic<int64_t> nnz(0);
-  parallel_for(npad, [&](int64_t p) {
-    const size_t b = (size_t)(p >> 6) * (kSlots * kSlice) + (size_t)(p & 63);
-    const int i = L.iperm_h[p];
-    for (int s = 0; s < kSlots; ++s) {
-      hc[b + s * kSlice] = (int)p;
-      hv[b + s * kSlice] = (i < 0 && s == kDiagSlot) ? 1.0 : 0.0;
-    }
-    if (i < 0 || bad) return;
-    const int ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
-    const int cnt = rows.nnz(i);
-    const int *rc = rows.cols(i);
-    const double *rv = rows.vals(i);
-    if (cnt < 1 || cnt > kSlots) { bad = 1; return; }
-    int last = -1;
-    bool diag = false;
-    for (int j = 0; j < cnt; ++j) {
-      const int col = rc[j];
-      if (col < 0 || col >= n) { bad = 1; return; }
-      const int dx = col % nx - ix, dy = (col / nx) % ny - iy, dz = col / (nx * ny) - iz;
-      if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) { bad = 1; return; }
-      const int s = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-      if (s <= last) { bad = 1; return; }
-      last = s;
-      if (s == kDiagSlot) diag = true;
-      hc[b + s * kSlice] = L.perm_h[col];
-      hv[b + s * kSlice] = rv[j];
-    }
-    if (!diag) { bad = 1; return; }
-    nnz += cnt;
-  });
-  if (bad) return false;
+  const int64_t chunks = (n + kUpRows - 1) / kUpRows;
+  for (int64_t q = 0; q < chunks; ++q) {
+    const int k = (int)(q & 1);
+    const int64_t i0 = q * kUpRows, m = std::min<int64_t>(kUpRows, n - i0);
+    if (q >= 2) HPCGX_HIP(c, hipEventSynchronize(c->up_ev[k]));  // host buffer free again
+    UpChunk h = up_view(c->up_host[k]), d = up_view(c->up_dev[k]);
+    parallel_for(m, [&](int64_t t) {
+      const int64_t i = i0 + t;
+      int cnt = rows.nnz(i);
+      if (cnt < 1 || cnt > kSlots) cnt = 0;  // row length 0: rejected on the device
+      h.nr[t] = (uint8_t)cnt;
+      std::memcpy(h.cols + t * kSlots, rows.cols(i), cnt * sizeof(int));
+      std::memcpy(h.vals + t * kSlots, rows.vals(i), cnt * sizeof(double));
+      nnz += cnt;
+    });
+    HPCGX_HIP(c, hipMemcpyAsync(d.vals, h.vals, (size_t)m * kSlots * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCGX_HIP(c, hipMemcpyAsync(d.cols, h.cols, (size_t)m * kSlots * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HPCGX_HIP(c, hipMemcpyAsync(d.nr, h.nr, (size_t)m, hipMemcpyHostToDevice, c->stream));
+    HPCGX_HIP(c, hipEventRecord(c->up_ev[k], c->stream));
+    k_pack_rows<<<nblk(m), kBlock, 0, c->stream>>>(L.ldev, n, d.nr, d.cols, d.vals, i0, m, L.perm, L.vals,
+                                                   c->up_err);
+    HPCGX_CHECK_LAUNCH(c);
+  }
+  int err[64];
+  HPCGX_HIP(c, hipMemcpyAsync(err, c->up_err, sizeof(err), hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  bool bad = false;
+  for (int e : err) bad = bad || e != 0;
+  if (bad) {
+    const size_t ms = (size_t)npad * kSlots;
+    dfree(c, L.cols, ms * sizeof(int));
+    dfree(c, L.vals, ms * sizeof(double));
+    dfree(c, L.perm, n * sizeof(int));
+    dfree(c, L.iperm, npad * sizeof(int));
+    L.cols = nullptr;
+    L.vals = nullptr;
+    L.perm = nullptr;
+    L.iperm = nullptr;
+    return HPCGX_OK;
+  }
+  L.perm_h.resize(n);
+  L.iperm_h.resize(npad);
+  HPCGX_HIP(c, hipMemcpy(L.perm_h.data(), L.perm, n * sizeof(int), hipMemcpyDeviceToHost));
+  HPCGX_HIP(c, hipMemcpy(L.iperm_h.data(), L.iperm, npad * sizeof(int), hipMemcpyDeviceToHost));
   L.nnz = nnz;
-  return true;
+  *stencil = true;
+  return HPCGX_OK;
 }
 
 // General layout for arbitrary rows (the reference requires every
@@ -482,24 +561,25 @@ static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, 
   L.ldev = to_dev(L.lay);
   if (L.lay.npad >= ((int64_t)1 << 31) && try_stencil)
     return set_err(c, HPCGX_EINVAL, "level_upload: too many rows for 32-bit columns");
-  std::vector<int> hc;
-  std::vector<double> hv;
-  std::vector<uint8_t> hd;
   int rc;
-  if (!try_stencil || !pack_stencil(L, rows, hc, hv)) {
+  bool stencil = false;
+  if (try_stencil && (rc = pack_stencil_device(c, L, rows, &stencil))) return rc;
+  if (!stencil) {
+    std::vector<int> hc;
+    std::vector<double> hv;
+    std::vector<uint8_t> hd;
     if ((rc = pack_general(c, L, rows, hc, hv, hd))) return rc;
-  }
-  const Layout &lay = L.lay;
-  const size_t m = (size_t)lay.npad * kSlots;
-  if ((rc = alloc_level(c, L))) return rc;
-  HPCGX_HIP(c, hipMemcpy(L.cols, hc.data(), m * sizeof(int), hipMemcpyHostToDevice));
-  HPCGX_HIP(c, hipMemcpy(L.vals, hv.data(), m * sizeof(double), hipMemcpyHostToDevice));
-  HPCGX_HIP(c, hipMemcpy(L.perm, L.perm_h.data(), lay.n * sizeof(int), hipMemcpyHostToDevice));
-  HPCGX_HIP(c, hipMemcpy(L.iperm, L.iperm_h.data(), lay.npad * sizeof(int), hipMemcpyHostToDevice));
-  if (lay.general) {
+    const Layout &lay = L.lay;
+    const size_t m = (size_t)lay.npad * kSlots;
+    if ((rc = alloc_level(c, L))) return rc;
+    HPCGX_HIP(c, hipMemcpy(L.cols, hc.data(), m * sizeof(int), hipMemcpyHostToDevice));
+    HPCGX_HIP(c, hipMemcpy(L.vals, hv.data(), m * sizeof(double), hipMemcpyHostToDevice));
+    HPCGX_HIP(c, hipMemcpy(L.perm, L.perm_h.data(), lay.n * sizeof(int), hipMemcpyHostToDevice));
+    HPCGX_HIP(c, hipMemcpy(L.iperm, L.iperm_h.data(), lay.npad * sizeof(int), hipMemcpyHostToDevice));
     if ((rc = dalloc(c, (void **)&L.dslot, lay.npad, false))) return rc;
     HPCGX_HIP(c, hipMemcpy(L.dslot, hd.data(), lay.npad, hipMemcpyHostToDevice));
   }
+  const Layout &lay = L.lay;
   if (f2c) {
     if (n_coarse < 1 || n_coarse > lay.n) return set_err(c, HPCGX_EINVAL, "level_upload: bad coarse row count for f2c");
     L.f2c_h.assign(f2c, f2c + n_coarse);
@@ -585,7 +665,10 @@ int ensure_ready(Ctx *c) {
     if (!L.x && (rc = dalloc(c, (void **)&L.x, L.lay.npad * sizeof(double)))) return rc;
   }
   const int64_t np = c->levels[0].lay.npad;
-  if ((rc = ensure_reduce(c, std::max<int64_t>((np + kBlock - 1) / kBlock, chunk_blocks(np))))) return rc;
+  // partials of one pass over level 0 (+ one tile per colour block for the
+  // r.z partials of the blocks' steps)
+  if ((rc = ensure_reduce(c, std::max<int64_t>((np + kBlock - 1) / kBlock, chunk_blocks(np)) + kMaxColours)))
+    return rc;
   if (c->cg_len < np) {
     HPCGX_HIP(c, hipStreamSynchronize(c->stream));
     for (double **b : {&c->cg_r, &c->cg_z, &c->cg_p, &c->cg_Ap}) {
@@ -846,6 +929,12 @@ int hpcgx_destroy(hpcgx_ctx *c) {
   (void)hipFree(c->scal);
   (void)hipFree(c->hist);
   if (c->hbuf) (void)hipHostFree(c->hbuf);
+  for (int k = 0; k < 2; ++k) {
+    if (c->up_host[k]) (void)hipHostFree(c->up_host[k]);
+    (void)hipFree(c->up_dev[k]);
+    if (c->up_ev[k]) (void)hipEventDestroy(c->up_ev[k]);
+  }
+  (void)hipFree(c->up_err);
   for (auto &ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -928,7 +1017,7 @@ int hpcgx_problem_generate(hpcgx_ctx *c, int nx, int ny, int nz, int levels, con
       }
     }
     k_generate<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.ldev, L.lay.npad, L.cols, L.vals, L.iperm,
-                                                          L.perm, kdev);
+                                                          L.perm, kdev, true);
     HPCGX_CHECK_LAUNCH(c);
     int64_t nnz = 1;
     for (const int d : {L.lay.nx, L.lay.ny, L.lay.nz}) nnz *= (int64_t)(3 * d - 2);

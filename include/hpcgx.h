@@ -12,9 +12,10 @@
  * hpcg_amd/dropin/ implement the reference signatures on top of them.
  *
  * Device vectors live in the context's row order: rows grouped by colour
- * (8 colours, the parity classes of the grid coordinates) and padded to 64
- * rows per colour.  hpcgx_vec_upload / hpcgx_vec_download convert from/to
- * the reference's natural row order.  Padding entries are always zero.
+ * (8 colours, the parity classes of the grid coordinates, for a 27-point
+ * stencil level; greedy colours for a general level) and padded to 64 rows
+ * per colour.  hpcgx_vec_upload / hpcgx_vec_download convert from/to the
+ * reference's natural row order.  Padding entries are zero after an upload.
  *
  * Calls are ordered on the context's stream (hpcgx_set_stream); calls that
  * return a scalar to the host (dot, cg) synchronise that stream.  A context

@@ -30,6 +30,17 @@
  *   mixed-precision MG / flexible CG     -> fp32 MG == fp64 MG to fp32
  *                                           rounding; converged solutions
  *                                           checked by an fp64 true residual
+ *   general levels (orc_create_rows,     -> natural-order spmv / symgs / mg /
+ *   orc_link)                               cg bit-exact against the
+ *                                           reference build with the same
+ *                                           rows written into its own
+ *                                           matrices (ref_replace_rows)
+ *   orc_colour_general                   -> equals the reference's greedy
+ *                                           example on symmetric patterns;
+ *                                           colour independence otherwise
+ *   coloured symgs on general levels     -> dense textbook Gauss-Seidel in
+ *                                           the colour order
+ *                                           (tests/test_oracle_general.py)
  */
 #ifndef HPCG_ORACLE_H
 #define HPCG_ORACLE_H
