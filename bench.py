@@ -13,12 +13,12 @@ timed on this host on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--grid N] [--gpus 1]
 
-The other configurations of the workload print their own JSON line:
-    --config 2   512^3: FP64 vs mixed-precision time to 1e-9, kernel HBM rates
-    --config 3   384^3 variable-coefficient operator, FP16 MG: forward error
-                 vs an FP64 solve at 1e-12
-    --config 4   128^3 adjoint through torch.autograd: backward/forward time,
-                 finite differences on 64 kappa entries, fp64 CPU gradient
+Other workloads print their own JSON line:
+    --case mixed     512^3: FP64 vs mixed-precision time to 1e-9, kernel HBM rates
+    --case kappa     384^3 variable-coefficient operator, FP16 MG: forward error
+                     vs an FP64 solve at 1e-12
+    --case adjoint   128^3 adjoint through torch.autograd: backward/forward time,
+                     finite differences on 64 kappa entries, fp64 CPU gradient
 """
 import argparse
 import json
@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-1e-9 solves")
     ap.add_argument("--cpu-iters", type=int, default=5)
-    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4))
+    ap.add_argument("--case", default="hpcg", choices=("hpcg", "mixed", "kappa", "adjoint"))
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="first copy 1 GiB with a plain copy kernel (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -91,12 +91,12 @@ def main():
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
-    if args.config == 2:
-        return config_mixed_512(args)
-    if args.config == 3:
-        return config_kappa_fp16(args)
-    if args.config == 4:
-        return config_adjoint(args)
+    if args.case == "mixed":
+        return case_mixed_512(args)
+    if args.case == "kappa":
+        return case_kappa_fp16(args)
+    if args.case == "adjoint":
+        return case_adjoint(args)
 
     import numpy as np
     import torch
@@ -251,8 +251,8 @@ def true_relres(ctx, db, dx, bnorm):
     return rr / bnorm
 
 
-def config_mixed_512(args):
-    """configs[2]: 512^3 HPCG problem, FP64 MG-PCG vs FP64-outer + FP32 (and
+def case_mixed_512(args):
+    """512^3 HPCG problem, FP64 MG-PCG vs FP64-outer + FP32 (and
     scaled FP16) MG to relative residual 1e-9, verified in FP64."""
     import numpy as np
     import torch
@@ -272,7 +272,7 @@ def config_mixed_512(args):
     b = hpcg_rhs(g, g, g)
     bn = float(np.linalg.norm(b))
     db, dx = ctx.vec(0, b), ctx.vec(0)
-    log(f"[bench c2] {g}^3 setup {setup:.1f} s, device {ctx.memory_bytes() / 1e9:.1f} GB")
+    log(f"[bench mixed] {g}^3 setup {setup:.1f} s, device {ctx.memory_bytes() / 1e9:.1f} GB")
     res = {}
     shapes = level_shapes(g, g, g)
     for name, variant, vmg, smg in (("fp64", hpcg_amd.HPCGX_CG_FP64, 8, 8),
@@ -288,7 +288,7 @@ def config_mixed_512(args):
         res[name] = {"seconds": round(o["seconds"], 3), "iterations": o["niters"],
                      "true_rel_residual": float(f"{rel:.4e}"), "ms_per_iteration": round(per_it * 1e3, 3),
                      "algorithmic_tb_s": round(by / per_it / 1e12, 3), "hpcg_model_gflops": round(mxp, 1)}
-        log(f"[bench c2] {name}: {o['niters']} it, {o['seconds']:.2f} s, rel res {rel:.3e}, "
+        log(f"[bench mixed] {name}: {o['niters']} it, {o['seconds']:.2f} s, rel res {rel:.3e}, "
             f"{per_it * 1e3:.2f} ms/it, {by / per_it / 1e12:.2f} TB/s")
     # the fp64 sets at this size, rated like the headline
     ctx.fill(dx, 0.0)
@@ -308,7 +308,7 @@ def config_mixed_512(args):
     ev1.synchronize()
     gs_bytes, nl = fine_gs_bytes(g)
     tl = ev0.elapsed_time(ev1) * 1e-3 / (3 * nl)
-    out = {"metric": METRIC, "config": {"workload": f"configs[2]: HPCG {g}^3, FP64 MG-PCG vs FP64-outer "
+    out = {"metric": METRIC, "config": {"workload": f"HPCG {g}^3, FP64 MG-PCG vs FP64-outer "
                                                     f"flexible PCG with FP32 / scaled-FP16 MG, to rel-res 1e-9",
                                         "grid": [g, g, g], "levels": 4},
            "value": res["fp64"]["seconds"] / res["mixed_fp32"]["seconds"], "unit": "x (time to 1e-9, FP64 / mixed FP32)",
@@ -329,8 +329,8 @@ def kappa_field(g, seed=2026):
     return kappa, b
 
 
-def config_kappa_fp16(args):
-    """configs[3]: 384^3 variable-coefficient operator (log kappa ~ N(0, 0.5^2),
+def case_kappa_fp16(args):
+    """384^3 variable-coefficient operator (log kappa ~ N(0, 0.5^2),
     b ~ U(-1,1), seeded); MG values stored as scaled FP16 with FP32
     accumulation, FP64 outer CG.  Forward error against an FP64 solve at 1e-12."""
     import numpy as np
@@ -354,7 +354,7 @@ def config_kappa_fp16(args):
     xref = ctx.download(dref)
     rows = {"fp64_1e-12": {"seconds": round(ref["seconds"], 3), "iterations": ref["niters"],
                            "true_rel_residual": float(f"{true_relres(ctx, db, dref, bn):.4e}")}}
-    log(f"[bench c3] fp64 reference: {ref['niters']} it {ref['seconds']:.2f} s")
+    log(f"[bench kappa] fp64 reference: {ref['niters']} it {ref['seconds']:.2f} s")
     for name, variant in (("mixed_fp16", hpcg_amd.HPCGX_CG_MIXED_FP16), ("mixed_fp32", hpcg_amd.HPCGX_CG_MIXED_FP32),
                           ("fp64", hpcg_amd.HPCGX_CG_FP64)):
         for tol in (1e-9, 1e-12):
@@ -364,8 +364,8 @@ def config_kappa_fp16(args):
             rel = true_relres(ctx, db, dx, bn)
             rows[f"{name}_{tol:g}"] = {"seconds": round(o["seconds"], 3), "iterations": o["niters"],
                                        "true_rel_residual": float(f"{rel:.4e}"), "forward_error": float(f"{fe:.4e}")}
-            log(f"[bench c3] {name} tol {tol:g}: {o['niters']} it {o['seconds']:.2f} s rel {rel:.2e} fwd err {fe:.2e}")
-    out = {"metric": METRIC, "config": {"workload": f"configs[3]: variable-coefficient 27-point operator {g}^3, "
+            log(f"[bench kappa] {name} tol {tol:g}: {o['niters']} it {o['seconds']:.2f} s rel {rel:.2e} fwd err {fe:.2e}")
+    out = {"metric": METRIC, "config": {"workload": f"variable-coefficient 27-point operator {g}^3, "
                                                     f"log kappa ~ N(0, 0.25), b ~ U(-1,1); MG in scaled FP16 values, "
                                                     f"FP32 arithmetic, FP64 outer PCG", "grid": [g, g, g]},
            "value": rows["mixed_fp16_1e-09"]["forward_error"], "unit": "forward error at rel-res 1e-9 (FP16 MG)",
@@ -374,8 +374,8 @@ def config_kappa_fp16(args):
     print(json.dumps(out), flush=True)
 
 
-def config_adjoint(args):
-    """configs[4]: adjoint through the solve at 128^3 (kappa, b as configs[3]);
+def case_adjoint(args):
+    """adjoint through the solve at 128^3 (kappa, b drawn as for the 384^3 case);
     L = 1/2 |x - xt|^2 with xt a seeded N(0,1) field smoothed by 3 SYMGS
     sweeps.  Forward and adjoint solves: mixed FP32-MG / FP64-outer to 1e-10."""
     import numpy as np
@@ -410,7 +410,7 @@ def config_adjoint(args):
         t_bwd = time.perf_counter() - t
     tim = dict(KappaSolve.last_timing)
     gk = k.grad.detach()
-    log(f"[bench c4] forward {t_fwd:.3f} s backward {t_bwd:.3f} s timing {tim}")
+    log(f"[bench adjoint] forward {t_fwd:.3f} s backward {t_bwd:.3f} s timing {tim}")
     # finite differences on 64 kappa entries, fp64 solves to 1e-13
     bd = torch.tensor(b, device="cuda")
     idx = rng.choice(g ** 3, 64, replace=False)
@@ -428,7 +428,7 @@ def config_adjoint(args):
     fd = np.array(fd)
     ga = gk[torch.tensor(idx, device="cuda")].cpu().numpy()
     fd_rel = float(np.max(np.abs(fd - ga)) / np.max(np.abs(ga)))
-    log(f"[bench c4] FD max rel err (vs max |g|) {fd_rel:.2e}")
+    log(f"[bench adjoint] FD max rel err (vs max |g|) {fd_rel:.2e}")
     # fp64 CPU reference gradient (the oracle, fp64 CG to 1e-13)
     cpu_rel = None
     try:
@@ -440,11 +440,11 @@ def config_adjoint(args):
         go = P.kappa_grad(lo, xo)
         cpu_rel = float(np.linalg.norm(gk.cpu().numpy() - go) / np.linalg.norm(go))
         b_rel = float(np.linalg.norm(bt.grad.cpu().numpy() - lo) / np.linalg.norm(lo))
-        log(f"[bench c4] vs fp64 CPU gradient: dL/dkappa rel {cpu_rel:.2e}, dL/db rel {b_rel:.2e}")
+        log(f"[bench adjoint] vs fp64 CPU gradient: dL/dkappa rel {cpu_rel:.2e}, dL/db rel {b_rel:.2e}")
     except Exception as e:
-        log(f"[bench c4] CPU gradient failed: {e}")
+        log(f"[bench adjoint] CPU gradient failed: {e}")
         b_rel = None
-    out = {"metric": METRIC, "config": {"workload": f"configs[4]: adjoint of the {g}^3 variable-coefficient solve "
+    out = {"metric": METRIC, "config": {"workload": f"adjoint of the {g}^3 variable-coefficient solve "
                                                     f"through torch.autograd.Function, mixed FP32-MG / FP64-outer "
                                                     f"PCG to 1e-10", "grid": [g, g, g]},
            "value": round(t_bwd / t_fwd, 3), "unit": "backward / forward time (autograd calls, incl. setup)",
