@@ -388,6 +388,8 @@ int hpcgx_cg(hpcgx_ctx *c, int variant, const double *b, double *x, int max_iter
   HPCGX_HIP(c, hipSetDevice(c->device));
   int rc = ensure_ready(c);
   if (rc) return rc;
+  const int64_t np = c->levels[0].lay.npad;
+  if ((rc = check_vec(c, b, np, "cg")) || (rc = check_vec(c, x, np, "cg"))) return rc;
   if (variant == HPCGX_CG_FP64)
     return cg_fp64(c, b, x, max_iter, tol, do_preconditioning != 0, niters, normr, normr0, times, hist);
   if (variant == HPCGX_CG_MIXED_FP32 || variant == HPCGX_CG_MIXED_FP16) {
@@ -408,6 +410,8 @@ int hpcgx_kappa_grad(hpcgx_ctx *c, const double *lam, const double *x, double *g
   if (rc) return rc;
   const Level &L = c->levels[0];
   if (L.lay.general) return set_err(c, HPCGX_EINVAL, "kappa_grad: the kappa operator is a 27-point stencil level");
+  for (const double *v : {lam, x, (const double *)g})
+    if ((rc = check_vec(c, v, L.lay.npad, "kappa_grad"))) return rc;
   k_kappa_grad<<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.iperm, lam, x, g, L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "hpcgx.h"
@@ -165,6 +166,7 @@ struct Ctx {
   std::vector<Level> levels;
   std::string err;
   double *partials = nullptr;  // per-block partial sums, 2 x the largest level-0 grid
+  double *folded = nullptr;    // second-level partials of large reductions (same allocation)
   int64_t reduce_cap = 0;      // largest grid the scratch covers
   double *scal = nullptr;      // device scalars [S_COUNT]
   double *hist = nullptr;      // device residual history
@@ -180,7 +182,7 @@ struct Ctx {
   uint8_t *up_host[2] = {}, *up_dev[2] = {};
   hipEvent_t up_ev[2] = {};
   int *up_err = nullptr;       // [64] per-lane error flags of the device packing
-  std::vector<void *> user_vecs;
+  std::unordered_map<const void *, int64_t> user_vecs;  // hpcgx_vec_alloc'd: elements
   int64_t bytes = 0;
   bool ready = false;
   int prepared = 0;  // bit 1: fp32 copies, bit 2: fp16 copies
@@ -199,6 +201,10 @@ int set_err(Ctx *c, int code, const std::string &msg);
 #define HPCGX_CHECK_LAUNCH(ctx) HPCGX_HIP(ctx, hipGetLastError())
 
 int dalloc(Ctx *c, void **p, size_t bytes, bool zero = true);
+// Rejects a vector made by hpcgx_vec_alloc that holds fewer than `elems`
+// doubles (a vector of another level, or of a problem since replaced).
+// Device pointers from elsewhere are the caller's to size.
+int check_vec(Ctx *c, const void *v, int64_t elems, const char *who);
 void dfree(Ctx *c, void *p, size_t bytes);
 int ensure_ready(Ctx *c);
 int ensure_stage(Ctx *c, int64_t n);

@@ -150,6 +150,32 @@ enum FinishOp : int {
 
 constexpr int kFinishThreads = 1024;
 
+// Reductions over more than kFoldAbove partials first fold each run of kFold
+// consecutive partials into one (one block per run), so the one-block
+// finishing kernel reads few values: its single CU would otherwise take
+// ~100 us over the 524288 partials of a 512^3 pass.
+constexpr int64_t kFold = 4096;
+constexpr int64_t kFoldAbove = 32768;
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_fold(const double *__restrict__ in, int64_t G, double *out) {
+  double v[NV];
+  const int64_t b0 = (int64_t)blockIdx.x * kFold;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double *p = in + (size_t)k * G;
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll 4
+    for (int64_t j = threadIdx.x; j < kFold; j += 2 * kBlock) {
+      const int64_t i = b0 + j;
+      if (i < G) a0 += p[i];
+      if (i + kBlock < G) a1 += p[i + kBlock];
+    }
+    v[k] = a0 + a1;
+  }
+  block_partials<NV>(v, out);
+}
+
 template <int NV>
 __global__ __launch_bounds__(kFinishThreads) void k_finish(const double *__restrict__ partials, int64_t G,
                                                            double *scal, int op) {

@@ -1,6 +1,8 @@
 // Kernel drivers: SpMV, coloured symmetric Gauss-Seidel, fused residual +
 // restriction, prolongation, the V-cycle in fp64 / fp32 / scaled fp16, and
 // the vector reductions.
+#include <initializer_list>
+
 #include "kernels.hpp"
 
 namespace hpcgx {
@@ -272,19 +274,33 @@ static int check_level(hpcgx_ctx *c, int level, const char *who) {
   return ensure_ready(c);
 }
 
+// Null pointers and vectors of the wrong size (those made by hpcgx_vec_alloc)
+// are refused before any launch.
+static int check_vecs(hpcgx_ctx *c, int64_t elems, const char *who, std::initializer_list<const void *> vs) {
+  for (const void *v : vs) {
+    if (!v) return set_err(c, HPCGX_EINVAL, std::string(who) + ": null vector");
+    int rc = check_vec(c, v, elems, who);
+    if (rc) return rc;
+  }
+  return HPCGX_OK;
+}
+
+static int64_t npad_of(hpcgx_ctx *c, int level) { return c->levels[level].lay.npad; }
+
 extern "C" {
 
 int hpcgx_spmv(hpcgx_ctx *c, int level, const double *x, double *y) {
   int rc = check_level(c, level, "spmv");
   if (rc) return rc;
-  if (!x || !y || x == y) return set_err(c, HPCGX_EINVAL, "spmv: x and y must be distinct device vectors");
+  if ((rc = check_vecs(c, npad_of(c, level), "spmv", {x, y}))) return rc;
+  if (x == y) return set_err(c, HPCGX_EINVAL, "spmv: x and y must be distinct device vectors");
   return spmv_fp64(c, level, x, y);
 }
 
 int hpcgx_symgs(hpcgx_ctx *c, int level, const double *r, double *x) {
   int rc = check_level(c, level, "symgs");
   if (rc) return rc;
-  if (!r || !x) return set_err(c, HPCGX_EINVAL, "symgs: null vector");
+  if ((rc = check_vecs(c, npad_of(c, level), "symgs", {r, x}))) return rc;
   return symgs_fp64(c, level, r, x);
 }
 
@@ -292,6 +308,9 @@ int hpcgx_restrict(hpcgx_ctx *c, int level, const double *r, const double *x, do
   int rc = check_level(c, level, "restrict");
   if (rc) return rc;
   if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "restrict: level has no coarse level");
+  if ((rc = check_vecs(c, npad_of(c, level), "restrict", {r, x})) ||
+      (rc = check_vecs(c, npad_of(c, level + 1), "restrict", {rc_out})))
+    return rc;
   const Level &L = c->levels[level];
   const int64_t nc = L.ninj;
   k_restrict<double, double, double, false, false><<<tiles(nc), kBlock, 0, c->stream>>>(
@@ -305,6 +324,9 @@ int hpcgx_prolong(hpcgx_ctx *c, int level, const double *xc, double *x) {
   int rc = check_level(c, level, "prolong");
   if (rc) return rc;
   if (level + 1 >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "prolong: level has no coarse level");
+  if ((rc = check_vecs(c, npad_of(c, level + 1), "prolong", {xc})) ||
+      (rc = check_vecs(c, npad_of(c, level), "prolong", {x})))
+    return rc;
   const Level &L = c->levels[level];
   const int64_t nc = L.ninj;
   k_prolong<double, double, false><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc, L.inj, nullptr,
@@ -316,14 +338,16 @@ int hpcgx_prolong(hpcgx_ctx *c, int level, const double *xc, double *x) {
 int hpcgx_mg(hpcgx_ctx *c, int level, const double *r, double *x) {
   int rc = check_level(c, level, "mg");
   if (rc) return rc;
-  if (!r || !x || r == x) return set_err(c, HPCGX_EINVAL, "mg: r and x must be distinct device vectors");
+  if ((rc = check_vecs(c, npad_of(c, level), "mg", {r, x}))) return rc;
+  if (r == x) return set_err(c, HPCGX_EINVAL, "mg: r and x must be distinct device vectors");
   return mg_fp64(c, level, r, x);
 }
 
 int hpcgx_mg_mixed(hpcgx_ctx *c, int precision, const double *r, double *x) {
   int rc = check_level(c, 0, "mg_mixed");
   if (rc) return rc;
-  if (!r || !x || r == x) return set_err(c, HPCGX_EINVAL, "mg_mixed: r and x must be distinct device vectors");
+  if ((rc = check_vecs(c, npad_of(c, 0), "mg_mixed", {r, x}))) return rc;
+  if (r == x) return set_err(c, HPCGX_EINVAL, "mg_mixed: r and x must be distinct device vectors");
   const int need = precision == HPCGX_PREC_FP32 ? 1 : precision == HPCGX_PREC_FP16 ? 2 : 0;
   if (!need) return set_err(c, HPCGX_EINVAL, "mg_mixed: precision must be HPCGX_PREC_FP32 or HPCGX_PREC_FP16");
   if (!(c->prepared & need)) return set_err(c, HPCGX_ESTATE, "mg_mixed: call hpcgx_prepare_precision first");
@@ -353,8 +377,9 @@ static int64_t span(hpcgx_ctx *c, int64_t n) {
 int hpcgx_dot(hpcgx_ctx *c, int64_t n, const double *x, const double *y, double *result) {
   int rc = check_n(c, n, "dot");
   if (rc) return rc;
-  if (!x || !y || !result) return set_err(c, HPCGX_EINVAL, "dot: null argument");
+  if (!result) return set_err(c, HPCGX_EINVAL, "dot: null argument");
   const int64_t m = span(c, n);
+  if ((rc = check_vecs(c, m, "dot", {x, y}))) return rc;
   k_dot<<<chunk_blocks(m), kBlock, 0, c->stream>>>(m, x, y, c->partials);
   HPCGX_CHECK_LAUNCH(c);
   if ((rc = finish(c, 1, chunk_blocks(m), F_DOT))) return rc;
@@ -368,8 +393,8 @@ int hpcgx_waxpby(hpcgx_ctx *c, int64_t n, double alpha, const double *x, double 
                  double *w) {
   int rc = check_n(c, n, "waxpby");
   if (rc) return rc;
-  if (!x || !y || !w) return set_err(c, HPCGX_EINVAL, "waxpby: null vector");
   const int64_t m = span(c, n);
+  if ((rc = check_vecs(c, m, "waxpby", {x, y, w}))) return rc;
   k_waxpby<<<tiles(m), kBlock, 0, c->stream>>>(m, alpha, x, beta, y, w);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
@@ -379,8 +404,9 @@ int hpcgx_waxpby_dot(hpcgx_ctx *c, int64_t n, double alpha, const double *x, dou
                      double *w, double *result) {
   int rc = check_n(c, n, "waxpby_dot");
   if (rc) return rc;
-  if (!x || !y || !w || !result) return set_err(c, HPCGX_EINVAL, "waxpby_dot: null argument");
+  if (!result) return set_err(c, HPCGX_EINVAL, "waxpby_dot: null argument");
   const int64_t m = span(c, n);
+  if ((rc = check_vecs(c, m, "waxpby_dot", {x, y, w}))) return rc;
   k_waxpby_dot<<<chunk_blocks(m), kBlock, 0, c->stream>>>(m, alpha, x, beta, y, w, c->partials);
   HPCGX_CHECK_LAUNCH(c);
   if ((rc = finish(c, 1, chunk_blocks(m), F_DOT))) return rc;

@@ -38,6 +38,14 @@ int dalloc(Ctx *c, void **p, size_t bytes, bool zero) {
   return HPCGX_OK;
 }
 
+int check_vec(Ctx *c, const void *v, int64_t elems, const char *who) {
+  auto it = c->user_vecs.find(v);
+  if (it != c->user_vecs.end() && it->second < elems)
+    return set_err(c, HPCGX_EINVAL, std::string(who) + ": vector holds " + std::to_string((long long)it->second) +
+                                        " entries, " + std::to_string((long long)elems) + " needed");
+  return HPCGX_OK;
+}
+
 void dfree(Ctx *c, void *p, size_t bytes) {
   if (!p) return;
   (void)hipFree(p);
@@ -108,23 +116,40 @@ static LayoutDev to_dev(const Layout &L) {
 }
 
 // Partial-sum scratch for reductions over grids of up to `grid` blocks.
+static size_t reduce_elems(int64_t grid) { return (size_t)(2 * grid + 16) + (size_t)(2 * (grid / kFold + 1) + 16); }
+
 int ensure_reduce(Ctx *c, int64_t grid) {
   if (c->reduce_cap >= grid) return HPCGX_OK;
-  if (c->partials) dfree(c, c->partials, 0);
+  if (c->partials) {
+    HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c, c->partials, reduce_elems(c->reduce_cap) * sizeof(double));
+  }
   c->partials = nullptr;
+  c->folded = nullptr;
   c->reduce_cap = 0;
-  int rc = dalloc(c, (void **)&c->partials, (size_t)(2 * grid + 16) * sizeof(double), false);
+  int rc = dalloc(c, (void **)&c->partials, reduce_elems(grid) * sizeof(double), false);
   if (rc) return rc;
+  c->folded = c->partials + 2 * grid + 16;
   c->reduce_cap = grid;
   return HPCGX_OK;
 }
 
 // One-block finishing kernel of a reduction over `grid` producer blocks.
 int finish(Ctx *c, int nv, int64_t grid, int op) {
+  const double *src = c->partials;
+  if (grid > kFoldAbove) {
+    const int64_t g2 = (grid + kFold - 1) / kFold;
+    if (nv == 1)
+      k_fold<1><<<(unsigned)g2, kBlock, 0, c->stream>>>(c->partials, grid, c->folded);
+    else
+      k_fold<2><<<(unsigned)g2, kBlock, 0, c->stream>>>(c->partials, grid, c->folded);
+    src = c->folded;
+    grid = g2;
+  }
   if (nv == 1)
-    k_finish<1><<<1, kFinishThreads, 0, c->stream>>>(c->partials, grid, c->scal, op);
+    k_finish<1><<<1, kFinishThreads, 0, c->stream>>>(src, grid, c->scal, op);
   else
-    k_finish<2><<<1, kFinishThreads, 0, c->stream>>>(c->partials, grid, c->scal, op);
+    k_finish<2><<<1, kFinishThreads, 0, c->stream>>>(src, grid, c->scal, op);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -917,7 +942,7 @@ int hpcgx_destroy(hpcgx_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &L : c->levels) free_level(c, L);
-  for (void *p : c->user_vecs) (void)hipFree(p);
+  for (auto &kv : c->user_vecs) (void)hipFree(const_cast<void *>(kv.first));
   (void)hipFree(c->cg_r);
   (void)hipFree(c->cg_z);
   (void)hipFree(c->cg_p);
@@ -1131,20 +1156,22 @@ double hpcgx_memory_bytes(const hpcgx_ctx *c) { return c ? (double)c->bytes : 0.
 int hpcgx_vec_alloc(hpcgx_ctx *c, int level, double **dvec) {
   if (!c || !dvec || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_alloc: bad argument");
   HPCGX_HIP(c, hipSetDevice(c->device));
-  int rc = dalloc(c, (void **)dvec, c->levels[level].lay.npad * sizeof(double));
+  const int64_t np = c->levels[level].lay.npad;
+  int rc = dalloc(c, (void **)dvec, np * sizeof(double));
   if (rc) return rc;
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
-  c->user_vecs.push_back(*dvec);
+  c->user_vecs[*dvec] = np;
   return HPCGX_OK;
 }
 
 int hpcgx_vec_free(hpcgx_ctx *c, double *dvec) {
   if (!c || !dvec) return HPCGX_EINVAL;
-  auto it = std::find(c->user_vecs.begin(), c->user_vecs.end(), (void *)dvec);
+  auto it = c->user_vecs.find(dvec);
   if (it == c->user_vecs.end()) return set_err(c, HPCGX_EINVAL, "vec_free: not a vector of this context");
+  const int64_t elems = it->second;
   c->user_vecs.erase(it);
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
-  HPCGX_HIP(c, hipFree(dvec));
+  dfree(c, dvec, elems * sizeof(double));
   return HPCGX_OK;
 }
 
@@ -1152,6 +1179,7 @@ int hpcgx_vec_upload(hpcgx_ctx *c, int level, double *dvec, const double *host, 
   if (!c || !dvec || !host || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_upload: bad argument");
   Level &L = c->levels[level];
   if (n != L.lay.n) return set_err(c, HPCGX_EINVAL, "vec_upload: length must equal the level's rows");
+  if (int rc0 = check_vec(c, dvec, L.lay.npad, "vec_upload")) return rc0;
   HPCGX_HIP(c, hipSetDevice(c->device));
   int rc = ensure_stage(c, n);
   if (rc) return rc;
@@ -1166,6 +1194,7 @@ int hpcgx_vec_download(hpcgx_ctx *c, int level, const double *dvec, double *host
   if (!c || !dvec || !host || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_download: bad argument");
   Level &L = c->levels[level];
   if (n != L.lay.n) return set_err(c, HPCGX_EINVAL, "vec_download: length must equal the level's rows");
+  if (int rc0 = check_vec(c, dvec, L.lay.npad, "vec_download")) return rc0;
   HPCGX_HIP(c, hipSetDevice(c->device));
   int rc = ensure_stage(c, n);
   if (rc) return rc;
@@ -1180,6 +1209,8 @@ int hpcgx_vec_from_natural(hpcgx_ctx *c, int level, double *dvec, const double *
   if (!c || !dvec || !dnat || level < 0 || level >= (int)c->levels.size() || dvec == dnat)
     return set_err(c, HPCGX_EINVAL, "vec_from_natural: bad argument");
   Level &L = c->levels[level];
+  if (int rc0 = check_vec(c, dvec, L.lay.npad, "vec_from_natural")) return rc0;
+  if (int rc0 = check_vec(c, dnat, L.lay.n, "vec_from_natural")) return rc0;
   HPCGX_HIP(c, hipSetDevice(c->device));
   k_gather_natural<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(dvec, dnat, L.iperm, L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
@@ -1190,6 +1221,8 @@ int hpcgx_vec_to_natural(hpcgx_ctx *c, int level, const double *dvec, double *dn
   if (!c || !dvec || !dnat || level < 0 || level >= (int)c->levels.size() || dvec == dnat)
     return set_err(c, HPCGX_EINVAL, "vec_to_natural: bad argument");
   Level &L = c->levels[level];
+  if (int rc0 = check_vec(c, dvec, L.lay.npad, "vec_to_natural")) return rc0;
+  if (int rc0 = check_vec(c, dnat, L.lay.n, "vec_to_natural")) return rc0;
   HPCGX_HIP(c, hipSetDevice(c->device));
   k_scatter_natural<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(dnat, dvec, L.perm, L.lay.n);
   HPCGX_CHECK_LAUNCH(c);
@@ -1199,6 +1232,7 @@ int hpcgx_vec_to_natural(hpcgx_ctx *c, int level, const double *dvec, double *dn
 int hpcgx_vec_fill(hpcgx_ctx *c, int level, double *dvec, double value) {
   if (!c || !dvec || level < 0 || level >= (int)c->levels.size()) return set_err(c, HPCGX_EINVAL, "vec_fill: bad argument");
   Level &L = c->levels[level];
+  if (int rc0 = check_vec(c, dvec, L.lay.npad, "vec_fill")) return rc0;
   HPCGX_HIP(c, hipSetDevice(c->device));
   k_fill<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(dvec, L.iperm, value, L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);

@@ -266,3 +266,21 @@ def test_mg_ignores_output_contents(hip, g):
     ctx.mg(0, dr, dz)
     assert np.array_equal(dz.download(), P.mg(r, mode=2))
     ctx.close()
+
+
+def test_wrong_size_vectors_are_refused(hip):
+    """A vector made for a coarser level, or for a smaller problem that was
+    since replaced, is refused before any launch."""
+    ctx = hip.Context(0)
+    ctx.generate(16, 16, 16)
+    small = ctx.vec(1)
+    big = ctx.vec(0)
+    with pytest.raises(hip.HpcgxError, match="vector holds"):
+        ctx.spmv(0, small, big)
+    with pytest.raises(hip.HpcgxError, match="vector holds"):
+        ctx.mg(0, big, small)
+    ctx.generate(32, 32, 32)
+    fresh = ctx.vec(0)
+    with pytest.raises(hip.HpcgxError, match="vector holds"):
+        ctx.cg(big, fresh, max_iter=1)
+    ctx.close()
