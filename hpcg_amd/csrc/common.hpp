@@ -210,8 +210,10 @@ int ensure_ready(Ctx *c);
 int ensure_stage(Ctx *c, int64_t n);
 
 // launch helpers (implemented in ops.hip)
-// rz: if given, the partials of r.x (result) are left there, *nrz of them
-int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz = nullptr, int64_t *nrz = nullptr);
+// rz: if given, the partials of r.x (result) are left there, *nrz of them.
+// untrusted_x: x is a caller's vector (its old contents must not leak in).
+int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz = nullptr, int64_t *nrz = nullptr,
+            bool untrusted_x = false);
 int mg_fp32(Ctx *c, int level, const float *r, float *x);
 int mg_fp16(Ctx *c, const float *r, float *x);
 int spmv_fp64(Ctx *c, int level, const double *x, double *y);

@@ -283,7 +283,7 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
 // profiles show the dominant launches separately from the coarse levels').
 // GEN: general level, the diagonal's slot is dslot[row] (stencil: slot 13).
 // gs_row is the update of one row; it returns r_i * x_i (new) for the fused r.z.
-template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI, bool GEN>
+template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI, bool GEN, bool SAFE>
 __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict__ vals,
                                          const XT *__restrict__ r, XT *__restrict__ x, int64_t row,
                                          const uint8_t *__restrict__ dslot) {
@@ -302,10 +302,12 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
       const int c = col_at<CM, FINE>(ci, row, slice, b, s);
       const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
       const ACC xc = (ACC)x[c];
-      // From x = 0, a zero entry (a neighbour outside the grid) may point at
-      // a row not written yet in this sweep, whose stale value is not
-      // trusted: it contributes 0 * 0, as the reference's zeroed x gives.
-      if constexpr (ZK >= 0) sum -= v * (v != (ACC)0 ? xc : (ACC)0);
+      // SAFE (from x = 0 into a caller's vector): a zero entry (a neighbour
+      // outside the grid) may point at a row not written yet in this sweep,
+      // whose old value is not trusted: it contributes 0 * 0, as the
+      // reference's zeroed x gives.  The solvers' own vectors always hold
+      // finite values, so their sweeps skip the select.
+      if constexpr (ZK >= 0 && SAFE) sum -= v * (v != (ACC)0 ? xc : (ACC)0);
       else sum -= v * xc;
     }
     return sum;
@@ -327,7 +329,8 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
 // post-smoother's last forward block and its backward steps); the block
 // also leaves its partial sum of r_i x_i in rz[blockIdx.x], so the solver's
 // r.z needs no pass of its own.
-template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false, bool RZ = false>
+template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false, bool RZ = false,
+          bool SAFE = false>
 __global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
                                                const XT *__restrict__ r, XT *__restrict__ x,
                                                int64_t row0, int64_t nrows,
@@ -338,13 +341,13 @@ __global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__rest
   if constexpr (RZ) {
     // every thread reaches the block reduction; rows past the block add 0
     double acc = 0.0;
-    if (t < nrows) acc = gs_row<VT, XT, ACC, ZK, FINE, CI, GEN>(ci, vals, r, x, row0 + t, dslot);
+    if (t < nrows) acc = gs_row<VT, XT, ACC, ZK, FINE, CI, GEN, SAFE>(ci, vals, r, x, row0 + t, dslot);
     const double v[1] = {acc};
     block_partials<1>(v, rz);
     return;
   }
   if (t >= nrows) return;
-  gs_row<VT, XT, ACC, ZK, FINE, CI, GEN>(ci, vals, r, x, row0 + t, dslot);
+  gs_row<VT, XT, ACC, ZK, FINE, CI, GEN, SAFE>(ci, vals, r, x, row0 + t, dslot);
 }
 
 // Residual at the injected rows and restriction, fused (ComputeMG_ref.cpp:49-51 +

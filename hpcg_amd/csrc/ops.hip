@@ -28,7 +28,7 @@ static ColIdx col_idx(const Level &L) {
 
 // rz (fp64 stencil levels only): the step also leaves the partials of r.x
 // over its rows there, tiles(rows) of them.
-template <class VT, class XT, class ACC, int ZK, bool FINE>
+template <class VT, class XT, class ACC, int ZK, bool FINE, bool SAFE = false>
 static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, int k, double *rz = nullptr) {
   const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
   if (nr <= 0) return;
@@ -51,9 +51,23 @@ static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, in
     }
   }
   if (ci.cbase)
-    k_gs<VT, XT, ACC, ZK, FINE, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr);
+    k_gs<VT, XT, ACC, ZK, FINE, true, false, false, SAFE><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr);
   else
-    k_gs<VT, XT, ACC, ZK, FINE, false><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr);
+    k_gs<VT, XT, ACC, ZK, FINE, false, false, false, SAFE><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr);
+}
+
+// The zero-start forward sweep of the default order: block k reads only the
+// blocks swept before it.
+template <class VT, class XT, class ACC, bool FINE, bool SAFE>
+static void zero_start_forward(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x) {
+  gs_block<VT, XT, ACC, 0, FINE, SAFE>(c, L, v, r, x, 0);
+  gs_block<VT, XT, ACC, 1, FINE, SAFE>(c, L, v, r, x, 1);
+  gs_block<VT, XT, ACC, 2, FINE, SAFE>(c, L, v, r, x, 2);
+  gs_block<VT, XT, ACC, 3, FINE, SAFE>(c, L, v, r, x, 3);
+  gs_block<VT, XT, ACC, 4, FINE, SAFE>(c, L, v, r, x, 4);
+  gs_block<VT, XT, ACC, 5, FINE, SAFE>(c, L, v, r, x, 5);
+  gs_block<VT, XT, ACC, 6, FINE, SAFE>(c, L, v, r, x, 6);
+  gs_block<VT, XT, ACC, 7, FINE, SAFE>(c, L, v, r, x, 7);
 }
 
 template <class XT>
@@ -77,23 +91,19 @@ static bool default_order(const Level &L) {
 // before it is written, otherwise it is zero-filled first.
 // rz: the steps that leave their rows final (forward step of the last block,
 // then the backward steps) write r.x partials at rz, *nrz of them in all.
+// untrusted_x: x is a caller's vector whose contents on a zero start may be
+// anything (not the solvers' own buffers, which always hold finite values).
 template <class VT, class XT, class ACC, bool FINE>
 static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start, double *rz = nullptr,
-                  int64_t *nrz = nullptr) {
+                  int64_t *nrz = nullptr, bool untrusted_x = false) {
   const VT *v = vals_of<VT>(L);
   if (zero_start && !default_order(L)) {  // other orders: zero x, then full steps
     k_zero<XT><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(x, L.lay.npad);
     zero_start = false;
   }
   if (zero_start) {
-    gs_block<VT, XT, ACC, 0, FINE>(c, L, v, r, x, 0);
-    gs_block<VT, XT, ACC, 1, FINE>(c, L, v, r, x, 1);
-    gs_block<VT, XT, ACC, 2, FINE>(c, L, v, r, x, 2);
-    gs_block<VT, XT, ACC, 3, FINE>(c, L, v, r, x, 3);
-    gs_block<VT, XT, ACC, 4, FINE>(c, L, v, r, x, 4);
-    gs_block<VT, XT, ACC, 5, FINE>(c, L, v, r, x, 5);
-    gs_block<VT, XT, ACC, 6, FINE>(c, L, v, r, x, 6);
-    gs_block<VT, XT, ACC, 7, FINE>(c, L, v, r, x, 7);
+    if (untrusted_x) zero_start_forward<VT, XT, ACC, FINE, true>(c, L, v, r, x);
+    else zero_start_forward<VT, XT, ACC, FINE, false>(c, L, v, r, x);
   } else {
     for (int k = 0; k + 1 < L.lay.ncol; ++k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
     const int last = L.lay.ncol - 1;
@@ -112,10 +122,10 @@ static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start, d
 
 template <class VT, class XT, class ACC>
 static int symgs_launch(Ctx *c, int l, const XT *r, XT *x, bool zero_start, double *rz = nullptr,
-                        int64_t *nrz = nullptr) {
+                        int64_t *nrz = nullptr, bool untrusted_x = false) {
   const Level &L = c->levels[l];
-  if (l == 0) sweep<VT, XT, ACC, true>(c, L, r, x, zero_start, rz, nrz);
-  else sweep<VT, XT, ACC, false>(c, L, r, x, zero_start, rz, nrz);
+  if (l == 0) sweep<VT, XT, ACC, true>(c, L, r, x, zero_start, rz, nrz, untrusted_x);
+  else sweep<VT, XT, ACC, false>(c, L, r, x, zero_start, rz, nrz, untrusted_x);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -123,11 +133,13 @@ static int symgs_launch(Ctx *c, int l, const XT *r, XT *x, bool zero_start, doub
 // V-cycle of ComputeMG_ref.cpp:39-63: x = 0; pre-smooth; residual + restrict;
 // recurse; prolong; post-smooth.  The coarsest level gets one smoothing step.
 // rz / nrz: r.x partials of the result from the post-smoother (see sweep);
-// fp64 stencil levels with a coarse level only.
+// fp64 stencil levels with a coarse level only.  untrusted_x: x is the
+// caller's (the coarse levels' vectors are the context's own).
 template <class VT, class XT, class ACC, bool SCALE>
-static int vcycle(Ctx *c, int l, const XT *r, XT *x, double *rz = nullptr, int64_t *nrz = nullptr) {
+static int vcycle(Ctx *c, int l, const XT *r, XT *x, double *rz = nullptr, int64_t *nrz = nullptr,
+                  bool untrusted_x = false) {
   const Level &L = c->levels[l];
-  int rc = symgs_launch<VT, XT, ACC>(c, l, r, x, true);
+  int rc = symgs_launch<VT, XT, ACC>(c, l, r, x, true, nullptr, nullptr, untrusted_x);
   if (rc || l + 1 >= (int)c->levels.size()) return rc;
   Level &C = c->levels[l + 1];
   XT *rc_v, *xc_v;
@@ -208,10 +220,10 @@ __global__ __launch_bounds__(kBlock) void k_waxpby_dot(int64_t n, double alpha,
 }
 
 
-int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz, int64_t *nrz) {
+int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz, int64_t *nrz, bool untrusted_x) {
   if (rz && (level + 1 >= (int)c->levels.size() || c->levels[level].lay.general))
     return set_err(c, HPCGX_EINVAL, "fused r.z needs a stencil level with a coarse level");
-  return vcycle<double, double, double, false>(c, level, r, x, rz, nrz);
+  return vcycle<double, double, double, false>(c, level, r, x, rz, nrz, untrusted_x);
 }
 
 int mg_fp32(Ctx *c, int level, const float *r, float *x) {
@@ -340,7 +352,7 @@ int hpcgx_mg(hpcgx_ctx *c, int level, const double *r, double *x) {
   if (rc) return rc;
   if ((rc = check_vecs(c, npad_of(c, level), "mg", {r, x}))) return rc;
   if (r == x) return set_err(c, HPCGX_EINVAL, "mg: r and x must be distinct device vectors");
-  return mg_fp64(c, level, r, x);
+  return mg_fp64(c, level, r, x, nullptr, nullptr, true);
 }
 
 int hpcgx_mg_mixed(hpcgx_ctx *c, int precision, const double *r, double *x) {
