@@ -53,6 +53,46 @@ def fine_gs_bytes(g):
     return sum(m[c] * 12 + 3 * n[c] * 8 for c in classes), len(classes)
 
 
+def per_kernel_rates(ctx, stream, g, dr, dx):
+    """Each hook's kernel on the finest level, timed with HIP events on the
+    bench stream (10 calls each after one untimed), against its algorithmic
+    bytes (hpcg_amd.models.kernel_bytes): achieved GB/s and the fraction of
+    the 8 TB/s HBM peak."""
+    import torch
+    from hpcg_amd.models import kernel_bytes, level_shapes
+    sh = level_shapes(g, g, g)
+    n, m = sh[0][0], sh[0][1]
+    nc = sh[1][0]
+    dy, dw, dxc, drc = ctx.vec(0), ctx.vec(0), ctx.vec(1), ctx.vec(1)
+    ctx.fill(dxc, 0.5)
+    cases = {
+        "spmv": (lambda: ctx.spmv(0, dx, dy), kernel_bytes("spmv", n, m)),
+        "symgs": (lambda: ctx.symgs(0, dr, dx), kernel_bytes("symgs", n, m)),
+        "restriction (fused residual)": (lambda: ctx.restrict(0, dr, dx, drc),
+                                         kernel_bytes("restrict", n, m, nc=nc, mc=m // 8)),
+        "prolongation": (lambda: ctx.prolong(0, dxc, dx), kernel_bytes("prolong", n, m, nc=nc)),
+        "waxpby": (lambda: ctx.waxpby(1.0, dx, -0.5, dr, dw), kernel_bytes("waxpby", n, m)),
+        # these two return the scalar to the host: each call includes that read
+        "dot (incl. host read)": (lambda: ctx.dot(dx, dr), kernel_bytes("dot", n, m)),
+        "waxpby+dot (incl. host read)": (lambda: ctx.waxpby_dot(1.0, dx, -0.5, dr, dw),
+                                         kernel_bytes("waxpby_dot", n, m)),
+    }
+    out = {}
+    for name, (call, by) in cases.items():
+        call()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(10):
+            call()
+        ev1.record(stream)
+        ev1.synchronize()
+        t = ev0.elapsed_time(ev1) * 1e-3 / 10
+        gbs = by / t / 1e9
+        out[name] = {"us": round(t * 1e6, 2), "gb_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"[bench] {name}: {t * 1e6:.1f} us, {gbs:.0f} GB/s algorithmic")
+    return out
+
+
 def cpu_baseline(g, iters):
     """The reference's CG_ref (OpenMP build, setup/Make.GCC_OMP flags) on the
     same grid for a few iterations, rated with the same FLOP model."""
@@ -176,6 +216,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_PER_LAUNCH.get(g),
             "avg_launch_us": round(t_launch * 1e6, 2), "bytes_per_launch": int(gs_bytes / nlaunch)}
     log(f"[bench] fine GS step: {t_launch * 1e6:.1f} us/launch, {achieved:.0f} GB/s")
+    kernels = per_kernel_rates(ctx, stream, g, dr, dx)
 
     # time to relative residual 1e-9 (true residual checked in fp64)
     tol_res = None
@@ -224,6 +265,7 @@ def main():
         "hbm_tb_s": round(tbs, 4),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "kernels": kernels,
         "time_to_rel_res_1e-9": tol_res,
     }
     print(json.dumps(out), flush=True)
