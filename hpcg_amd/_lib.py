@@ -34,6 +34,7 @@ SIGNATURES = {
     "hpcgx_level_upload_packed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _u8p, _ip, _dp, _vp]),
     "hpcgx_level_upload_rows": (C.c_int, [_vp, C.c_int, C.c_int64, _u8p, _ip, _dp, _vp, C.c_int64]),
     "hpcgx_level_colouring": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
+    "hpcgx_colour_rows": (C.c_int, [C.c_int64, _u8p, _ip, _ip, _ip, C.POINTER(C.c_int)]),
     "hpcgx_problem_generate": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "hpcgx_level_info": (C.c_int, [_vp, C.c_int, _i64p]),
     "hpcgx_level_set_diagonal": (C.c_int, [_vp, C.c_int, _dp]),
@@ -83,6 +84,19 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def colour_rows(nnz_row, cols):
+    """hpcgx_colour_rows: (colour per row, breadth-first order, colours) of
+    rows packed [n, 27] (host only)."""
+    nnz_row = np.ascontiguousarray(nnz_row, np.uint8)
+    cols = np.ascontiguousarray(cols, np.int32).reshape(-1)
+    n = len(nnz_row)
+    col, order, nc = np.empty(n, np.int32), np.empty(n, np.int32), C.c_int(0)
+    rc = lib().hpcgx_colour_rows(n, nnz_row, cols, col, order, C.byref(nc))
+    if rc != 0:
+        raise HpcgxError(f"hpcgx_colour_rows: {ERRORS.get(rc, rc)}")
+    return col, order, nc.value
 
 
 def _ptr(v):

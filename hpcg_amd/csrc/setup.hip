@@ -447,15 +447,100 @@ static int pack_stencil_device(Ctx *c, Level &L, RowAcc rows, bool *stencil) {
   return HPCGX_OK;
 }
 
+// Colouring of a general level (rows already validated): the breadth-first
+// order and the first-fit colours in it.  False if more than kMaxColours
+// colours would be needed.
+template <class RowAcc>
+static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std::vector<int> &colour, int &ncol) {
+  // transpose: the rows j != i that hold column i
+  std::vector<int64_t> ts(n + 1, 0);
+  for (int64_t j = 0; j < n; ++j) {
+    const int cnt = rows.nnz(j);
+    const int *rc = rows.cols(j);
+    for (int k = 0; k < cnt; ++k)
+      if (rc[k] != j) ts[rc[k] + 1]++;
+  }
+  for (int64_t i = 0; i < n; ++i) ts[i + 1] += ts[i];
+  std::vector<int> tj(std::max<int64_t>(ts[n], 1));
+  {
+    std::vector<int64_t> fill(ts.begin(), ts.end() - 1);
+    for (int64_t j = 0; j < n; ++j) {
+      const int cnt = rows.nnz(j);
+      const int *rc = rows.cols(j);
+      for (int k = 0; k < cnt; ++k)
+        if (rc[k] != j) tj[fill[rc[k]]++] = (int)j;
+    }
+  }
+  // Breadth-first order: components in order of their smallest row s; a
+  // pass from s finds the last row it reaches, the pass from that row gives
+  // the order (starting far out: on a renumbered 27-point stencil the first
+  // fit below then needs the minimum, 8 colours, where natural order needs 17).
+  // Neighbours of row i: its entries in row order (diagonal skipped), then the
+  // rows holding column i in ascending order.
+  order.clear();
+  order.reserve(n);
+  {
+    std::vector<char> seen(n, 0);
+    std::vector<int> probe;
+    probe.reserve(1024);
+    std::vector<int64_t> stamp(n, -1);
+    auto visit = [&](int i, auto &&take) {
+      const int cnt = rows.nnz(i);
+      const int *rc = rows.cols(i);
+      for (int k = 0; k < cnt; ++k)
+        if (rc[k] != i) take(rc[k]);
+      for (int64_t t = ts[i]; t < ts[i + 1]; ++t) take(tj[t]);
+    };
+    for (int64_t s0 = 0; s0 < n; ++s0) {
+      if (seen[s0]) continue;
+      probe.clear();
+      probe.push_back((int)s0);
+      stamp[s0] = s0;
+      for (size_t h = 0; h < probe.size(); ++h)
+        visit(probe[h], [&](int j) {
+          if (stamp[j] != s0) { stamp[j] = s0; probe.push_back(j); }
+        });
+      const int start = probe.back();
+      size_t head = order.size();
+      order.push_back(start);
+      seen[start] = 1;
+      while (head < order.size())
+        visit(order[head++], [&](int j) {
+          if (!seen[j]) { seen[j] = 1; order.push_back(j); }
+        });
+    }
+  }
+  // First fit in that order: row i takes the smallest colour no coloured
+  // neighbour holds (for either direction of a coupling).
+  colour.assign(n, -1);
+  int64_t mark[kMaxColours + 1];
+  for (auto &m : mark) m = -1;
+  ncol = 0;
+  for (const int i : order) {
+    const int cnt = rows.nnz(i);
+    const int *rc = rows.cols(i);
+    for (int k = 0; k < cnt; ++k)
+      if (rc[k] != i && colour[rc[k]] >= 0) mark[colour[rc[k]]] = i;
+    for (int64_t t = ts[i]; t < ts[i + 1]; ++t)
+      if (colour[tj[t]] >= 0) mark[colour[tj[t]]] = i;
+    int pick = 0;
+    while (mark[pick] == i) ++pick;
+    if (pick >= kMaxColours) return false;
+    colour[i] = pick;
+    ncol = std::max(ncol, pick + 1);
+  }
+  return true;
+}
+
 // General layout for arbitrary rows (the reference requires every
 // optimisation to work for general sparse matrices, GenerateProblem.cpp:47-49).
-// Colouring: first fit in row order over the symmetrised pattern -- row i
-// takes the smallest colour not held by a j < i with a_ij != 0 or a_ji != 0.
-// On structurally symmetric rows this is the reference's greedy example
-// (OptimizeProblem.cpp:43-79); the transpose term keeps each colour free of
+// Colouring: first fit, like the reference's greedy example
+// (OptimizeProblem.cpp:43-79), but in a breadth-first order and over the
+// symmetrised pattern (a_ij != 0 or a_ji != 0), so each colour stays free of
 // couplings when the pattern is not symmetric.  Blocks in colour order, rows
-// ascending inside a block, slot j = the row's j-th entry, so sums run in the
-// reference's order (ComputeSPMV_ref.cpp:67-68, ComputeSYMGS_ref.cpp:74-77).
+// in the breadth-first order inside a block, slot j = the row's j-th entry,
+// so sums run in the reference's order (ComputeSPMV_ref.cpp:67-68,
+// ComputeSYMGS_ref.cpp:74-77).
 template <class RowAcc>
 static int pack_general(Ctx *c, Level &L, RowAcc rows, std::vector<int> &hc, std::vector<double> &hv,
                         std::vector<uint8_t> &hd) {
@@ -474,43 +559,11 @@ static int pack_general(Ctx *c, Level &L, RowAcc rows, std::vector<int> &hc, std
   if (bad_row >= 0)
     return set_err(c, HPCGX_EINVAL, "level_upload: row " + std::to_string((long long)bad_row) +
                                          " has no diagonal, a column out of range, or not 1..27 entries");
-  // transpose: rows j < i that hold column i
-  std::vector<int64_t> ts(n + 1, 0);
-  for (int64_t j = 0; j < n; ++j) {
-    const int cnt = rows.nnz(j);
-    const int *rc = rows.cols(j);
-    for (int k = 0; k < cnt; ++k)
-      if (rc[k] > j) ts[rc[k] + 1]++;
-  }
-  for (int64_t i = 0; i < n; ++i) ts[i + 1] += ts[i];
-  std::vector<int> tj(std::max<int64_t>(ts[n], 1));
-  {
-    std::vector<int64_t> fill(ts.begin(), ts.end() - 1);
-    for (int64_t j = 0; j < n; ++j) {
-      const int cnt = rows.nnz(j);
-      const int *rc = rows.cols(j);
-      for (int k = 0; k < cnt; ++k)
-        if (rc[k] > j) tj[fill[rc[k]]++] = (int)j;
-    }
-  }
-  std::vector<int> colour(n);
-  int64_t mark[kMaxColours + 1];
-  for (auto &m : mark) m = -1;
+  std::vector<int> order, colour;
   int ncol = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    const int cnt = rows.nnz(i);
-    const int *rc = rows.cols(i);
-    for (int k = 0; k < cnt; ++k)
-      if (rc[k] < i) mark[colour[rc[k]]] = i;
-    for (int64_t t = ts[i]; t < ts[i + 1]; ++t) mark[colour[tj[t]]] = i;
-    int pick = 0;
-    while (mark[pick] == i) ++pick;
-    if (pick >= kMaxColours)
-      return set_err(c, HPCGX_EINVAL, "level_upload: greedy colouring needs more than " +
-                                          std::to_string(kMaxColours) + " colours");
-    colour[i] = pick;
-    ncol = std::max(ncol, pick + 1);
-  }
+  if (!colour_general(rows, n, order, colour, ncol))
+    return set_err(c, HPCGX_EINVAL, "level_upload: greedy colouring needs more than " +
+                                        std::to_string(kMaxColours) + " colours");
   Layout &lay = L.lay;
   lay.general = true;
   lay.ncol = ncol;
@@ -524,14 +577,18 @@ static int pack_general(Ctx *c, Level &L, RowAcc rows, std::vector<int> &hc, std
   lay.npad = lay.start[ncol];
   if (lay.npad >= ((int64_t)1 << 31)) return set_err(c, HPCGX_EINVAL, "level_upload: too many rows for 32-bit columns");
   const int64_t npad = lay.npad;
+  // rows inside a block in the breadth-first order: neighbouring rows land
+  // near each other in every block, so the entries a tile of rows gathers
+  // are close together whatever the caller's numbering (the order inside a
+  // colour changes no result: rows of one colour are not coupled)
   L.perm_h.assign(n, 0);
   L.iperm_h.assign(npad, -1);
   {
     std::vector<int64_t> fill(lay.start, lay.start + ncol);
-    for (int64_t i = 0; i < n; ++i) {
+    for (const int i : order) {
       const int64_t p = fill[colour[i]]++;
       L.perm_h[i] = (int)p;
-      L.iperm_h[p] = (int)i;
+      L.iperm_h[p] = i;
     }
   }
   const size_t m = (size_t)npad * kSlots;
@@ -1094,6 +1151,22 @@ int hpcgx_level_info(const hpcgx_ctx *c, int level, int64_t *out) {
   out[5] = L.lay.nz;
   out[6] = L.lay.npad * kSlots;
   out[7] = (int64_t)c->levels.size();
+  return HPCGX_OK;
+}
+
+int hpcgx_colour_rows(int64_t n, const uint8_t *nnz_row, const int *cols, int *colour, int *order, int *ncolours) {
+  if (n < 1 || n >= ((int64_t)1 << 31) || !nnz_row || !cols || !colour || !ncolours) return HPCGX_EINVAL;
+  for (int64_t i = 0; i < n; ++i) {
+    if (nnz_row[i] < 1 || nnz_row[i] > kSlots) return HPCGX_EINVAL;
+    for (int k = 0; k < nnz_row[i]; ++k)
+      if (cols[i * kSlots + k] < 0 || cols[i * kSlots + k] >= n) return HPCGX_EINVAL;
+  }
+  std::vector<int> ord, col;
+  int nc = 0;
+  if (!colour_general(PackedRows{nnz_row, cols, nullptr}, n, ord, col, nc)) return HPCGX_EINVAL;
+  std::memcpy(colour, col.data(), n * sizeof(int));
+  if (order) std::memcpy(order, ord.data(), n * sizeof(int));
+  *ncolours = nc;
   return HPCGX_OK;
 }
 

@@ -112,6 +112,14 @@ int hpcgx_level_info(const hpcgx_ctx *ctx, int level, int64_t *out);
  * NULL) the colour of every row in natural order; colours are swept
  * 0..ncolours-1 forward and back down to 0. */
 int hpcgx_level_colouring(const hpcgx_ctx *ctx, int level, int *ncolours, int *general, int *colour);
+/* The colouring the general layout gives rows packed as for
+ * hpcgx_level_upload_rows (OptimizeProblem.cpp:42-97 restated for general
+ * patterns): first fit in a breadth-first order over the symmetrised
+ * pattern, the order starting from a far row of each component.  colour[n],
+ * order[n] (may be NULL): the breadth-first sequence.  Host only; needs no
+ * device or context. */
+int hpcgx_colour_rows(int64_t n, const uint8_t *nnz_row, const int *cols, int *colour, int *order,
+                      int *ncolours);
 /* Replaces the diagonal of a level (ReplaceMatrixDiagonal, SparseMatrix.hpp:139-145);
  * diag is host, natural order.  Lower-precision copies are rebuilt. */
 int hpcgx_level_set_diagonal(hpcgx_ctx *ctx, int level, const double *diag);

@@ -226,19 +226,22 @@ void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours) {
   *ncolours = total;
 }
 
-/* First-fit greedy colouring in row order over the symmetrised pattern:
- * row i avoids the colours of every j < i with a_ij != 0 or a_ji != 0.  On a
- * structurally symmetric matrix this is exactly orc_colour_greedy (the
- * reference's OptimizeProblem.cpp:43-79 example, which looks at a_ij only);
- * the transpose term keeps colours independent for unsymmetric patterns. */
-void orc_colour_general(const orc_level *L, int *colour, int *ncolours) {
+/* Colouring of a general level: first fit in breadth-first order over the
+ * symmetrised pattern.  The neighbours of row i are its own entries (row
+ * order, the diagonal skipped) followed by the rows j that hold column i
+ * (ascending j).  Components are taken in order of their smallest row s; a
+ * breadth-first pass from s finds the last row it reaches, and the pass from
+ * that row gives the order (a far row to start from: on a renumbered 27-point
+ * stencil this finds a corner, and the first fit then needs 8 colours where
+ * natural order needs 17).  Row i takes the smallest colour not held by an
+ * already coloured neighbour.  order[] (may be NULL) receives the sequence. */
+void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, int *order) {
   const int n = L->n;
-  /* lower neighbours from the transpose: rows j < i whose row holds i */
   int *tcnt = (int *)calloc((size_t)n + 1, sizeof(int));
   for (int j = 0; j < n; ++j)
     for (int k = 0; k < L->nnz_row[j]; ++k) {
       const int c = L->cols[(int64_t)j * NSLOT + k];
-      if (c > j) tcnt[c + 1]++;
+      if (c != j) tcnt[c + 1]++;
     }
   for (int i = 0; i < n; ++i) tcnt[i + 1] += tcnt[i];
   int *tj = (int *)malloc((size_t)(tcnt[n] > 0 ? tcnt[n] : 1) * sizeof(int));
@@ -247,24 +250,68 @@ void orc_colour_general(const orc_level *L, int *colour, int *ncolours) {
   for (int j = 0; j < n; ++j)
     for (int k = 0; k < L->nnz_row[j]; ++k) {
       const int c = L->cols[(int64_t)j * NSLOT + k];
-      if (c > j) tj[fill[c]++] = j;
+      if (c != j) tj[fill[c]++] = j;
     }
+  int *seq = (int *)malloc((size_t)n * sizeof(int));
+  int *probe = (int *)malloc((size_t)n * sizeof(int));
+  int *stamp = (int *)malloc((size_t)n * sizeof(int));
+  char *seen = (char *)calloc((size_t)n, 1);
+  for (int i = 0; i < n; ++i) stamp[i] = -1;
+  int len = 0;
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (seen[s0]) continue;
+    /* probe pass from s0 (its component is wholly unseen) */
+    int h = 0, t = 0;
+    probe[t++] = s0;
+    stamp[s0] = s0;
+    while (h < t) {
+      const int i = probe[h++];
+      for (int k = 0; k < L->nnz_row[i]; ++k) {
+        const int c = L->cols[(int64_t)i * NSLOT + k];
+        if (c != i && stamp[c] != s0) { stamp[c] = s0; probe[t++] = c; }
+      }
+      for (int q = tcnt[i]; q < tcnt[i + 1]; ++q)
+        if (stamp[tj[q]] != s0) { stamp[tj[q]] = s0; probe[t++] = tj[q]; }
+    }
+    const int start = probe[t - 1];
+    /* the pass that gives the order */
+    int head = len;
+    seq[len++] = start;
+    seen[start] = 1;
+    while (head < len) {
+      const int i = seq[head++];
+      for (int k = 0; k < L->nnz_row[i]; ++k) {
+        const int c = L->cols[(int64_t)i * NSLOT + k];
+        if (c != i && !seen[c]) { seen[c] = 1; seq[len++] = c; }
+      }
+      for (int q = tcnt[i]; q < tcnt[i + 1]; ++q)
+        if (!seen[tj[q]]) { seen[tj[q]] = 1; seq[len++] = tj[q]; }
+    }
+  }
   int *mark = (int *)malloc(((size_t)n + 1) * sizeof(int));
   for (int i = 0; i <= n; ++i) mark[i] = -1;
+  for (int i = 0; i < n; ++i) colour[i] = -1;
   int total = 0;
-  for (int i = 0; i < n; ++i) {
+  for (int q = 0; q < n; ++q) {
+    const int i = seq[q];
     for (int k = 0; k < L->nnz_row[i]; ++k) {
       const int c = L->cols[(int64_t)i * NSLOT + k];
-      if (c < i) mark[colour[c]] = i;
+      if (c != i && colour[c] >= 0) mark[colour[c]] = i;
     }
-    for (int t = tcnt[i]; t < tcnt[i + 1]; ++t) mark[colour[tj[t]]] = i;
+    for (int r = tcnt[i]; r < tcnt[i + 1]; ++r)
+      if (colour[tj[r]] >= 0) mark[colour[tj[r]]] = i;
     int pick = 0;
     while (mark[pick] == i) ++pick;
     colour[i] = pick;
     if (pick + 1 > total) total = pick + 1;
   }
   *ncolours = total;
-  free(tcnt); free(tj); free(fill); free(mark);
+  if (order) memcpy(order, seq, (size_t)n * sizeof(int));
+  free(tcnt); free(tj); free(fill); free(seq); free(probe); free(stamp); free(seen); free(mark);
+}
+
+void orc_colour_general(const orc_level *L, int *colour, int *ncolours) {
+  orc_colour_general_order(L, colour, ncolours, NULL);
 }
 
 /* Rows grouped by colour (colours swept in ascending order, rows ascending

@@ -35,9 +35,11 @@
  *                                           reference build with the same
  *                                           rows written into its own
  *                                           matrices (ref_replace_rows)
- *   orc_colour_general                   -> equals the reference's greedy
- *                                           example on symmetric patterns;
- *                                           colour independence otherwise
+ *   orc_colour_general                   -> a first fit in its breadth-first
+ *                                           order (restated in numpy) with
+ *                                           independent colours; the order
+ *                                           is a breadth-first sequence of
+ *                                           every component
  *   coloured symgs on general levels     -> dense textbook Gauss-Seidel in
  *                                           the colour order
  *                                           (tests/test_oracle_general.py)
@@ -88,6 +90,7 @@ void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours);
 orc_level *orc_create_rows(int n, const unsigned char *nnz_row, const int *cols, const double *vals);
 int orc_link(orc_level *fine, orc_level *coarse, const int *f2c);
 void orc_colour_general(const orc_level *L, int *colour, int *ncolours);
+void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, int *order);
 int orc_set_colouring(orc_level *L, const int *colour, int ncolours);
 int orc_ncolours(const orc_level *L);
 const int *orc_colour_rows(const orc_level *L);   /* rows grouped by colour */

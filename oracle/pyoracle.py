@@ -75,6 +75,7 @@ def lib():
         L.orc_link.restype = C.c_int
         L.orc_link.argtypes = [vp, vp, _ip]
         L.orc_colour_general.argtypes = [vp, _ip, C.POINTER(C.c_int)]
+        L.orc_colour_general_order.argtypes = [vp, _ip, C.POINTER(C.c_int), _ip]
         L.orc_set_colouring.restype = C.c_int
         L.orc_set_colouring.argtypes = [vp, _ip, C.c_int]
         L.orc_ncolours.restype = C.c_int
@@ -177,12 +178,13 @@ class Level:
         lib().orc_colour_greedy(self.h, col, C.byref(nc))
         return col, nc.value
 
-    def general_colours(self):
-        """orc_colour_general: (colour per row, number of colours)."""
+    def general_colours(self, with_order=False):
+        """orc_colour_general: (colour per row, number of colours[, order])."""
         col = np.empty(self.n, np.int32)
+        order = np.empty(self.n, np.int32)
         nc = C.c_int(0)
-        lib().orc_colour_general(self.h, col, C.byref(nc))
-        return col, nc.value
+        lib().orc_colour_general_order(self.h, col, C.byref(nc), order)
+        return (col, nc.value, order) if with_order else (col, nc.value)
 
     def set_colouring(self, colour):
         colour = np.ascontiguousarray(colour, np.int32)

@@ -1,0 +1,32 @@
+"""The general layout's colouring, computed by the library on the host
+(hpcgx_colour_rows: no device needed), is the oracle's (orc_colour_general,
+pinned in test_oracle_general.py) colour for colour and row for row."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import general_matrices as gm
+
+CASES = {"permuted_hpcg": lambda: gm.permuted_hpcg(16, seed=1),
+         "random_spd": lambda: gm.random_spd(seed=2),
+         "unsymmetric": lambda: gm.unsymmetric(sizes=(3000, 400), seed=3)}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_library_colouring_equals_oracle(hip, case):
+    levels = CASES[case]()
+    P = po.Problem.from_rows(levels)
+    for k, (nr, cols, _, _) in enumerate(levels):
+        col, order, nc = hip._lib.colour_rows(nr, cols)
+        want, wnc, word = P.level(k).general_colours(with_order=True)
+        assert nc == wnc
+        assert np.array_equal(col, want)
+        assert np.array_equal(order, word)
+
+
+def test_library_colouring_rejects_bad_rows(hip):
+    nr = np.array([1, 2], np.uint8)
+    cols = np.zeros((2, 27), np.int32)
+    cols[1, :2] = [1, 5]  # column out of range
+    with pytest.raises(hip.HpcgxError):
+        hip._lib.colour_rows(nr, cols)

@@ -4,9 +4,12 @@ any pattern, coloured by orc_colour_general.
 * natural-order kernels, the V-cycle and CG bit for bit against the
   reference compiled from its own sources, after its levels' rows are
   replaced in place by the same rows (ref_replace_rows);
-* the colouring: equal to the reference's greedy example
-  (OptimizeProblem.cpp:43-79) on structurally symmetric patterns, and free of
-  same-colour couplings also for unsymmetric ones;
+* the colouring: free of same-colour couplings (also for unsymmetric
+  patterns), a first fit in its own order (every smaller colour is held by an
+  earlier neighbour), that order a breadth-first sequence, and 8 colours -- the
+  minimum, a 2x2x2 cell is a clique -- on a randomly renumbered 27-point
+  stencil, where the reference's natural-order greedy example
+  (OptimizeProblem.cpp:43-79) needs 17;
 * coloured symmetric Gauss-Seidel against a dense textbook statement in the
   colour order.
 """
@@ -45,16 +48,53 @@ def test_general_oracle_matches_reference_build(case):
     assert np.array_equal(got["x"], want["x"])
 
 
+def _sym_adjacency(lv):
+    nr, cols, _, _ = lv.arrays()
+    adj = [set() for _ in range(lv.n)]
+    for i in range(lv.n):
+        for c in cols[i, :nr[i]]:
+            if c != i:
+                adj[i].add(int(c))
+                adj[int(c)].add(i)
+    return adj
+
+
+def _assert_first_fit_bfs(lv, col, order):
+    adj = _sym_adjacency(lv)
+    n = lv.n
+    assert sorted(order) == list(range(n))
+    pos = np.empty(n, np.int64)
+    pos[order] = np.arange(n)
+    last_parent = -1
+    for q, i in enumerate(order):
+        earlier = [j for j in adj[i] if pos[j] < q]
+        used = {col[j] for j in earlier}
+        assert col[i] not in used
+        assert all(c in used for c in range(col[i]))       # first fit
+        if earlier:                                          # breadth-first:
+            parent = min(pos[j] for j in earlier)            # parents come in
+            assert parent >= last_parent                     # order
+            last_parent = parent
+        else:
+            last_parent = q  # a new component starts here
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_general_colouring_is_reference_greedy_on_symmetric_patterns(case):
+def test_general_colouring_is_first_fit_in_breadth_first_order(case):
     P = po.Problem.from_rows(CASES[case]())
     for k in range(4):
         lv = P.level(k)
-        col, nc = lv.general_colours()
-        ref_col, ref_nc = lv.greedy_colours()
-        assert nc == ref_nc and np.array_equal(col, ref_col)
-        assert lv.ncolours() == nc
+        col, nc, order = lv.general_colours(with_order=True)
+        assert lv.ncolours() == nc == col.max() + 1
         _assert_independent(lv, col)
+        _assert_first_fit_bfs(lv, col, order)
+
+
+def test_general_colouring_of_renumbered_stencil_is_minimal():
+    P = po.Problem.from_rows(gm.permuted_hpcg(16, levels=2, seed=3))
+    assert P.general_colours()[1] == 8
+    assert P.level(1).general_colours()[1] == 8
+    assert P.greedy_colours()[1] > 8   # the natural-order example does worse here
 
 
 def _assert_independent(lv, col):
@@ -67,8 +107,9 @@ def _assert_independent(lv, col):
 def test_general_colouring_unsymmetric_pattern():
     levels = gm.unsymmetric(sizes=(600, 100), seed=4)
     P = po.Problem.from_rows(levels)
-    col, nc = P.general_colours()
+    col, nc, order = P.general_colours(with_order=True)
     _assert_independent(P, col)
+    _assert_first_fit_bfs(P, col, order)
     # the pattern really is unsymmetric
     A = gm.dense(levels[0])
     assert np.any((A != 0) != (A.T != 0))
