@@ -471,10 +471,12 @@ static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std:
         if (rc[k] != j) tj[fill[rc[k]]++] = (int)j;
     }
   }
-  // Breadth-first order: components in order of their smallest row s; a
-  // pass from s finds the last row it reaches, the pass from that row gives
-  // the order (starting far out: on a renumbered 27-point stencil the first
-  // fit below then needs the minimum, 8 colours, where natural order needs 17).
+  // Breadth-first order: components in order of their smallest row s.  A far
+  // row to start from: a pass from x = s ends on row fx; while a pass from fx
+  // ends on a deeper level than the pass from x did, x moves to fx (at most 8
+  // moves).  The pass from the final fx gives the order (on a renumbered
+  // 27-point stencil it starts at a corner, and the first fit below needs
+  // the minimum, 8 colours, where natural order needs 17).
   // Neighbours of row i: its entries in row order (diagonal skipped), then the
   // rows holding column i in ascending order.
   order.clear();
@@ -484,6 +486,8 @@ static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std:
     std::vector<int> probe;
     probe.reserve(1024);
     std::vector<int64_t> stamp(n, -1);
+    std::vector<int> level(n, 0);
+    int64_t stamp_id = 0;
     auto visit = [&](int i, auto &&take) {
       const int cnt = rows.nnz(i);
       const int *rc = rows.cols(i);
@@ -491,16 +495,34 @@ static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std:
         if (rc[k] != i) take(rc[k]);
       for (int64_t t = ts[i]; t < ts[i + 1]; ++t) take(tj[t]);
     };
+    // a breadth-first pass over s's component: its last row and that row's level
+    auto probe_pass = [&](int s, int &last, int &last_level) {
+      const int64_t id = stamp_id++;
+      probe.clear();
+      probe.push_back(s);
+      stamp[s] = id;
+      level[s] = 0;
+      for (size_t h = 0; h < probe.size(); ++h) {
+        const int i = probe[h];
+        visit(i, [&](int j) {
+          if (stamp[j] != id) { stamp[j] = id; level[j] = level[i] + 1; probe.push_back(j); }
+        });
+      }
+      last = probe.back();
+      last_level = level[last];
+    };
     for (int64_t s0 = 0; s0 < n; ++s0) {
       if (seen[s0]) continue;
-      probe.clear();
-      probe.push_back((int)s0);
-      stamp[s0] = s0;
-      for (size_t h = 0; h < probe.size(); ++h)
-        visit(probe[h], [&](int j) {
-          if (stamp[j] != s0) { stamp[j] = s0; probe.push_back(j); }
-        });
-      const int start = probe.back();
+      // a far row: from x to its last row fx, moving on while that deepens
+      int fx, ex, fy, ey;
+      probe_pass((int)s0, fx, ex);
+      for (int it = 0; it < 8; ++it) {
+        probe_pass(fx, fy, ey);
+        if (ey <= ex) break;
+        ex = ey;
+        fx = fy;
+      }
+      const int start = fx;
       size_t head = order.size();
       order.push_back(start);
       seen[start] = 1;

@@ -226,14 +226,37 @@ void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours) {
   *ncolours = total;
 }
 
+/* Breadth-first pass from s over its component (rows marked with a fresh
+ * stamp): the last row reached and its level. */
+static void probe_bfs(const orc_level *L, const int *tcnt, const int *tj, int s, int *q, int *stamp,
+                      int *level, int *stamp_id, int *last, int *last_level) {
+  const int id = (*stamp_id)++;
+  int h = 0, t = 0;
+  q[t++] = s;
+  stamp[s] = id;
+  level[s] = 0;
+  while (h < t) {
+    const int i = q[h++];
+    for (int k = 0; k < L->nnz_row[i]; ++k) {
+      const int c = L->cols[(int64_t)i * NSLOT + k];
+      if (c != i && stamp[c] != id) { stamp[c] = id; level[c] = level[i] + 1; q[t++] = c; }
+    }
+    for (int r = tcnt[i]; r < tcnt[i + 1]; ++r)
+      if (stamp[tj[r]] != id) { stamp[tj[r]] = id; level[tj[r]] = level[i] + 1; q[t++] = tj[r]; }
+  }
+  *last = q[t - 1];
+  *last_level = level[q[t - 1]];
+}
+
 /* Colouring of a general level: first fit in breadth-first order over the
  * symmetrised pattern.  The neighbours of row i are its own entries (row
  * order, the diagonal skipped) followed by the rows j that hold column i
- * (ascending j).  Components are taken in order of their smallest row s; a
- * breadth-first pass from s finds the last row it reaches, and the pass from
- * that row gives the order (a far row to start from: on a renumbered 27-point
- * stencil this finds a corner, and the first fit then needs 8 colours where
- * natural order needs 17).  Row i takes the smallest colour not held by an
+ * (ascending j).  Components are taken in order of their smallest row s.  A
+ * far row to start from: a breadth-first pass from x = s gives its last row
+ * fx; while a pass from fx ends on a deeper level than the pass from x did,
+ * x moves to fx (at most 8 moves).  The pass from the final fx gives the
+ * order.  On a renumbered 27-point stencil this starts at a corner and the
+ * first fit needs 8 colours, where natural order needs 17.  Row i takes the smallest colour not held by an
  * already coloured neighbour.  order[] (may be NULL) receives the sequence. */
 void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, int *order) {
   const int n = L->n;
@@ -255,25 +278,24 @@ void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, in
   int *seq = (int *)malloc((size_t)n * sizeof(int));
   int *probe = (int *)malloc((size_t)n * sizeof(int));
   int *stamp = (int *)malloc((size_t)n * sizeof(int));
+  int *level = (int *)malloc((size_t)n * sizeof(int));
   char *seen = (char *)calloc((size_t)n, 1);
+  int stamp_id = 0;
   for (int i = 0; i < n; ++i) stamp[i] = -1;
   int len = 0;
   for (int s0 = 0; s0 < n; ++s0) {
     if (seen[s0]) continue;
-    /* probe pass from s0 (its component is wholly unseen) */
-    int h = 0, t = 0;
-    probe[t++] = s0;
-    stamp[s0] = s0;
-    while (h < t) {
-      const int i = probe[h++];
-      for (int k = 0; k < L->nnz_row[i]; ++k) {
-        const int c = L->cols[(int64_t)i * NSLOT + k];
-        if (c != i && stamp[c] != s0) { stamp[c] = s0; probe[t++] = c; }
-      }
-      for (int q = tcnt[i]; q < tcnt[i + 1]; ++q)
-        if (stamp[tj[q]] != s0) { stamp[tj[q]] = s0; probe[t++] = tj[q]; }
+    /* probe passes (s0's component is wholly unseen): from x to its last
+     * row fx; move to fx while that raises the last row's level */
+    int fx, ex, fy, ey;
+    probe_bfs(L, tcnt, tj, s0, probe, stamp, level, &stamp_id, &fx, &ex);
+    for (int it = 0; it < 8; ++it) {
+      probe_bfs(L, tcnt, tj, fx, probe, stamp, level, &stamp_id, &fy, &ey);
+      if (ey <= ex) break;
+      ex = ey;
+      fx = fy;
     }
-    const int start = probe[t - 1];
+    const int start = fx;
     /* the pass that gives the order */
     int head = len;
     seq[len++] = start;
@@ -307,7 +329,7 @@ void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, in
   }
   *ncolours = total;
   if (order) memcpy(order, seq, (size_t)n * sizeof(int));
-  free(tcnt); free(tj); free(fill); free(seq); free(probe); free(stamp); free(seen); free(mark);
+  free(tcnt); free(tj); free(fill); free(seq); free(probe); free(stamp); free(level); free(seen); free(mark);
 }
 
 void orc_colour_general(const orc_level *L, int *colour, int *ncolours) {

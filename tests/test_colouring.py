@@ -30,3 +30,11 @@ def test_library_colouring_rejects_bad_rows(hip):
     cols[1, :2] = [1, 5]  # column out of range
     with pytest.raises(hip.HpcgxError):
         hip._lib.colour_rows(nr, cols)
+
+
+@pytest.mark.parametrize("g,seed", [(16, 1), (16, 5), (24, 7), (32, 11)])
+def test_renumbered_stencil_gets_eight_colours(hip, g, seed):
+    """8 is the minimum for the 27-point stencil (a 2x2x2 cell is a clique);
+    first fit in natural order of a random numbering needs about 17."""
+    nr, cols, _, _ = gm.permuted_hpcg(g, levels=1, seed=seed)[0]
+    assert hip._lib.colour_rows(nr, cols)[2] == 8
