@@ -34,7 +34,7 @@ SIGNATURES = {
     "hpcgx_level_upload_packed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _u8p, _ip, _dp, _vp]),
     "hpcgx_level_upload_rows": (C.c_int, [_vp, C.c_int, C.c_int64, _u8p, _ip, _dp, _vp, C.c_int64]),
     "hpcgx_level_colouring": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
-    "hpcgx_colour_rows": (C.c_int, [C.c_int64, _u8p, _ip, _ip, _ip, C.POINTER(C.c_int)]),
+    "hpcgx_colour_rows": (C.c_int, [C.c_int64, _u8p, _ip, _vp, C.c_int64, _ip, _ip, C.POINTER(C.c_int)]),
     "hpcgx_problem_generate": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "hpcgx_level_info": (C.c_int, [_vp, C.c_int, _i64p]),
     "hpcgx_level_set_diagonal": (C.c_int, [_vp, C.c_int, _dp]),
@@ -86,14 +86,17 @@ def lib():
     return _lib
 
 
-def colour_rows(nnz_row, cols):
+def colour_rows(nnz_row, cols, f2c=None):
     """hpcgx_colour_rows: (colour per row, breadth-first order, colours) of
-    rows packed [n, 27] (host only)."""
+    rows packed [n, 27], colours ordered by injected rows when f2c is given
+    (host only)."""
     nnz_row = np.ascontiguousarray(nnz_row, np.uint8)
     cols = np.ascontiguousarray(cols, np.int32).reshape(-1)
     n = len(nnz_row)
+    fa = None if f2c is None else np.ascontiguousarray(f2c, np.int32)
+    fp = None if fa is None else fa.ctypes.data_as(C.c_void_p)
     col, order, nc = np.empty(n, np.int32), np.empty(n, np.int32), C.c_int(0)
-    rc = lib().hpcgx_colour_rows(n, nnz_row, cols, col, order, C.byref(nc))
+    rc = lib().hpcgx_colour_rows(n, nnz_row, cols, fp, 0 if fa is None else len(fa), col, order, C.byref(nc))
     if rc != 0:
         raise HpcgxError(f"hpcgx_colour_rows: {ERRORS.get(rc, rc)}")
     return col, order, nc.value

@@ -554,6 +554,22 @@ static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std:
   return true;
 }
 
+// Colour order with an injection: colours sorted by how many injected rows
+// (f2c targets) they hold, ascending, ties by colour, so the colour of the
+// injected rows is swept last forward and first backward (swept last
+// backward, its residual would be zero and the V-cycle would lose its
+// coarse correction).
+static void order_colours(int64_t n, int ncol, std::vector<int> &colour, const int *f2c, int64_t nc) {
+  std::vector<int64_t> cnt(ncol, 0);
+  for (int64_t i = 0; i < nc; ++i)
+    if (f2c[i] >= 0 && f2c[i] < n) cnt[colour[f2c[i]]]++;
+  std::vector<int> rank(ncol, 0);
+  for (int k = 0; k < ncol; ++k)
+    for (int j = 0; j < ncol; ++j)
+      if (cnt[j] < cnt[k] || (cnt[j] == cnt[k] && j < k)) rank[k]++;
+  for (auto &cl : colour) cl = rank[cl];
+}
+
 // General layout for arbitrary rows (the reference requires every
 // optimisation to work for general sparse matrices, GenerateProblem.cpp:47-49).
 // Colouring: first fit, like the reference's greedy example
@@ -564,8 +580,8 @@ static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std:
 // so sums run in the reference's order (ComputeSPMV_ref.cpp:67-68,
 // ComputeSYMGS_ref.cpp:74-77).
 template <class RowAcc>
-static int pack_general(Ctx *c, Level &L, RowAcc rows, std::vector<int> &hc, std::vector<double> &hv,
-                        std::vector<uint8_t> &hd) {
+static int pack_general(Ctx *c, Level &L, RowAcc rows, const int *f2c, int64_t n_coarse, std::vector<int> &hc,
+                        std::vector<double> &hv, std::vector<uint8_t> &hd) {
   const int64_t n = L.lay.n;
   std::atomic<int64_t> bad_row(-1);
   parallel_for(n, [&](int64_t i) {
@@ -586,6 +602,7 @@ static int pack_general(Ctx *c, Level &L, RowAcc rows, std::vector<int> &hc, std
   if (!colour_general(rows, n, order, colour, ncol))
     return set_err(c, HPCGX_EINVAL, "level_upload: greedy colouring needs more than " +
                                         std::to_string(kMaxColours) + " colours");
+  if (f2c) order_colours(n, ncol, colour, f2c, n_coarse);
   Layout &lay = L.lay;
   lay.general = true;
   lay.ncol = ncol;
@@ -672,7 +689,7 @@ static int upload_level(Ctx *c, int level, int nx, int ny, int nz, RowAcc rows, 
     std::vector<int> hc;
     std::vector<double> hv;
     std::vector<uint8_t> hd;
-    if ((rc = pack_general(c, L, rows, hc, hv, hd))) return rc;
+    if ((rc = pack_general(c, L, rows, f2c, n_coarse, hc, hv, hd))) return rc;
     const Layout &lay = L.lay;
     const size_t m = (size_t)lay.npad * kSlots;
     if ((rc = alloc_level(c, L))) return rc;
@@ -1176,7 +1193,8 @@ int hpcgx_level_info(const hpcgx_ctx *c, int level, int64_t *out) {
   return HPCGX_OK;
 }
 
-int hpcgx_colour_rows(int64_t n, const uint8_t *nnz_row, const int *cols, int *colour, int *order, int *ncolours) {
+int hpcgx_colour_rows(int64_t n, const uint8_t *nnz_row, const int *cols, const int *f2c, int64_t n_coarse,
+                      int *colour, int *order, int *ncolours) {
   if (n < 1 || n >= ((int64_t)1 << 31) || !nnz_row || !cols || !colour || !ncolours) return HPCGX_EINVAL;
   for (int64_t i = 0; i < n; ++i) {
     if (nnz_row[i] < 1 || nnz_row[i] > kSlots) return HPCGX_EINVAL;
@@ -1186,6 +1204,10 @@ int hpcgx_colour_rows(int64_t n, const uint8_t *nnz_row, const int *cols, int *c
   std::vector<int> ord, col;
   int nc = 0;
   if (!colour_general(PackedRows{nnz_row, cols, nullptr}, n, ord, col, nc)) return HPCGX_EINVAL;
+  if (f2c) {
+    if (n_coarse < 0) return HPCGX_EINVAL;
+    order_colours(n, nc, col, f2c, n_coarse);
+  }
   std::memcpy(colour, col.data(), n * sizeof(int));
   if (order) std::memcpy(order, ord.data(), n * sizeof(int));
   *ncolours = nc;

@@ -115,11 +115,13 @@ int hpcgx_level_colouring(const hpcgx_ctx *ctx, int level, int *ncolours, int *g
 /* The colouring the general layout gives rows packed as for
  * hpcgx_level_upload_rows (OptimizeProblem.cpp:42-97 restated for general
  * patterns): first fit in a breadth-first order over the symmetrised
- * pattern, the order starting from a far row of each component.  colour[n],
+ * pattern, the order starting from a far row of each component; with an
+ * injection f2c[n_coarse] (may be NULL), colours are then sorted by how many
+ * injected rows they hold, so that colour is swept last forward.  colour[n];
  * order[n] (may be NULL): the breadth-first sequence.  Host only; needs no
  * device or context. */
-int hpcgx_colour_rows(int64_t n, const uint8_t *nnz_row, const int *cols, int *colour, int *order,
-                      int *ncolours);
+int hpcgx_colour_rows(int64_t n, const uint8_t *nnz_row, const int *cols, const int *f2c, int64_t n_coarse,
+                      int *colour, int *order, int *ncolours);
 /* Replaces the diagonal of a level (ReplaceMatrixDiagonal, SparseMatrix.hpp:139-145);
  * diag is host, natural order.  Lower-precision copies are rebuilt. */
 int hpcgx_level_set_diagonal(hpcgx_ctx *ctx, int level, const double *diag);

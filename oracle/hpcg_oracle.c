@@ -421,6 +421,30 @@ orc_level *orc_create_rows(int n, const unsigned char *nnz_row, const int *cols,
   return L;
 }
 
+/* Colour order of a general level with an injection: colours sorted by how
+ * many injected rows (f2c targets) they hold, ascending and stable, so the
+ * colour of the injected rows is swept last forward and first backward (if
+ * it were swept last backward its residual would be zero and the V-cycle
+ * would lose its coarse correction).  colour[] is relabelled in place. */
+void orc_order_colours(int n, int ncolours, int *colour, const int *f2c, int nc) {
+  int *cnt = (int *)calloc((size_t)ncolours, sizeof(int));
+  int *rank = (int *)malloc((size_t)ncolours * sizeof(int));
+  for (int i = 0; i < nc; ++i) cnt[colour[f2c[i]]]++;
+  /* rank of colour k: colours with fewer injected rows first, ties by k */
+  for (int k = 0; k < ncolours; ++k) {
+    rank[k] = 0;
+    for (int j = 0; j < ncolours; ++j)
+      if (cnt[j] < cnt[k] || (cnt[j] == cnt[k] && j < k)) rank[k]++;
+  }
+  for (int i = 0; i < n; ++i) colour[i] = rank[colour[i]];
+  free(cnt); free(rank);
+}
+
+void orc_level_colours(const orc_level *L, int *colour) {
+  for (int k = 0; k < L->ncol; ++k)
+    for (int t = L->cstart[k]; t < L->cstart[k + 1]; ++t) colour[L->crow[t]] = k;
+}
+
 /* Makes coarse the next level of fine with the injection f2c[n_coarse]
  * (MGData.hpp:56-63: the coarse operator and f2cOperator).  fine takes
  * ownership of coarse.  -1 if f2c is out of range or not one-to-one (the
@@ -441,6 +465,13 @@ int orc_link(orc_level *fine, orc_level *coarse, const int *f2c) {
   fine->xc = (double *)calloc((size_t)nc, sizeof(double));
   fine->rc32 = (float *)calloc((size_t)nc, sizeof(float));
   fine->xc32 = (float *)calloc((size_t)nc, sizeof(float));
+  if (fine->general) {  /* the injected rows' colour goes last */
+    int *colour = (int *)malloc((size_t)fine->n * sizeof(int));
+    orc_level_colours(fine, colour);
+    orc_order_colours(fine->n, fine->ncol, colour, f2c, nc);
+    orc_set_colouring(fine, colour, fine->ncol);
+    free(colour);
+  }
   return 0;
 }
 

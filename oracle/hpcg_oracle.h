@@ -84,13 +84,17 @@ void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours);
 /* General (unstructured) levels.  orc_create_rows builds one level from
  * arbitrary rows (<= 27 entries each, row order kept, one diagonal) and
  * colours it with orc_colour_general; orc_link attaches a coarse level with
- * an injection f2c.  orc_set_colouring replaces the colouring of one level
+ * an injection f2c (and reorders the fine level's colours so the colour of
+ * the injected rows comes last).  orc_set_colouring replaces the colouring of one level
  * (colours swept in ascending order); -1 if two rows of one colour are
  * coupled. */
 orc_level *orc_create_rows(int n, const unsigned char *nnz_row, const int *cols, const double *vals);
 int orc_link(orc_level *fine, orc_level *coarse, const int *f2c);
 void orc_colour_general(const orc_level *L, int *colour, int *ncolours);
 void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, int *order);
+/* colours sorted by their count of injected rows (orc_link applies it) */
+void orc_order_colours(int n, int ncolours, int *colour, const int *f2c, int nc);
+void orc_level_colours(const orc_level *L, int *colour);  /* the level's colouring now */
 int orc_set_colouring(orc_level *L, const int *colour, int ncolours);
 int orc_ncolours(const orc_level *L);
 const int *orc_colour_rows(const orc_level *L);   /* rows grouped by colour */

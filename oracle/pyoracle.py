@@ -78,6 +78,7 @@ def lib():
         L.orc_colour_general_order.argtypes = [vp, _ip, C.POINTER(C.c_int), _ip]
         L.orc_set_colouring.restype = C.c_int
         L.orc_set_colouring.argtypes = [vp, _ip, C.c_int]
+        L.orc_level_colours.argtypes = [vp, _ip]
         L.orc_ncolours.restype = C.c_int
         L.orc_ncolours.argtypes = [vp]
         L.orc_f32_to_f16.restype = C.c_uint16
@@ -193,6 +194,12 @@ class Level:
 
     def ncolours(self):
         return lib().orc_ncolours(self.h)
+
+    def colours(self):
+        """The colouring the level is swept in now (after orc_link's ordering)."""
+        col = np.empty(self.n, np.int32)
+        lib().orc_level_colours(self.h, col)
+        return col
 
 
 class Problem(Level):

@@ -16,11 +16,12 @@ CASES = {"permuted_hpcg": lambda: gm.permuted_hpcg(16, seed=1),
 def test_library_colouring_equals_oracle(hip, case):
     levels = CASES[case]()
     P = po.Problem.from_rows(levels)
-    for k, (nr, cols, _, _) in enumerate(levels):
-        col, order, nc = hip._lib.colour_rows(nr, cols)
-        want, wnc, word = P.level(k).general_colours(with_order=True)
-        assert nc == wnc
-        assert np.array_equal(col, want)
+    for k, (nr, cols, _, f2c) in enumerate(levels):
+        col, order, nc = hip._lib.colour_rows(nr, cols, f2c)
+        lv = P.level(k)
+        _, wnc, word = lv.general_colours(with_order=True)
+        assert nc == wnc == lv.ncolours()
+        assert np.array_equal(col, lv.colours())   # after the injected-rows ordering
         assert np.array_equal(order, word)
 
 

@@ -60,8 +60,8 @@ def test_general_layout_matrix_and_colouring(cases, name):
     for k, (nr, cols, vals, _) in enumerate(levels):
         col, nc, general = ctx.colouring(k)
         assert general
-        want, wnc = P.level(k).general_colours()
-        assert nc == wnc and np.array_equal(col, want)
+        lv = P.level(k)
+        assert nc == lv.ncolours() and np.array_equal(col, lv.colours())
         enr, ecols, evals = ctx.export_level(k)
         assert np.array_equal(enr, nr)
         mask = np.arange(27)[None, :] < nr[:, None]

@@ -90,6 +90,26 @@ def test_general_colouring_is_first_fit_in_breadth_first_order(case):
         _assert_first_fit_bfs(lv, col, order)
 
 
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_injected_colours_are_swept_last(case):
+    """orc_link orders a general level's colours by how many injected rows
+    they hold (ascending): the colour with most injected rows is swept last
+    forward and first backward.  The ordering only relabels the first-fit
+    colouring (same classes)."""
+    levels = CASES[case]()
+    P = po.Problem.from_rows(levels)
+    for k in range(3):
+        lv = P.level(k)
+        col = lv.colours()
+        f2c = levels[k][3]
+        counts = np.bincount(col[f2c], minlength=lv.ncolours())
+        assert np.all(np.diff(counts) >= 0)
+        raw, nc = lv.general_colours()
+        pairs = set(zip(raw.tolist(), col.tolist()))
+        assert len(pairs) == nc   # a relabelling: classes unchanged
+        _assert_independent(lv, col)
+
+
 def test_general_colouring_of_renumbered_stencil_is_minimal():
     P = po.Problem.from_rows(gm.permuted_hpcg(16, levels=2, seed=3))
     assert P.general_colours()[1] == 8
