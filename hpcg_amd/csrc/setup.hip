@@ -452,17 +452,40 @@ static int pack_stencil_device(Ctx *c, Level &L, RowAcc rows, bool *stencil) {
 // colours would be needed.
 template <class RowAcc>
 static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std::vector<int> &colour, int &ncol) {
-  // transpose: the rows j != i that hold column i
-  std::vector<int64_t> ts(n + 1, 0);
-  for (int64_t j = 0; j < n; ++j) {
-    const int cnt = rows.nnz(j);
-    const int *rc = rows.cols(j);
-    for (int k = 0; k < cnt; ++k)
-      if (rc[k] != j) ts[rc[k] + 1]++;
-  }
-  for (int64_t i = 0; i < n; ++i) ts[i + 1] += ts[i];
-  std::vector<int> tj(std::max<int64_t>(ts[n], 1));
-  {
+  // A symmetric pattern (each off-diagonal entry (i, c) has its (c, i), no
+  // column twice in a row) needs no transposed lists: the rows holding column
+  // i are then row i's own columns, already marked when such a list would be
+  // walked, so the results are the same without them.
+  std::atomic<bool> asym(false);
+  parallel_for(n, [&](int64_t i) {
+    const int cnt = rows.nnz(i);
+    const int *rc = rows.cols(i);
+    for (int k = 0; k < cnt && !asym.load(std::memory_order_relaxed); ++k) {
+      const int c = rc[k];
+      if (c == i) continue;
+      for (int q = 0; q < k; ++q)
+        if (rc[q] == c) asym = true;
+      const int cc = rows.nnz(c);
+      const int *cr = rows.cols(c);
+      bool back = false;
+      for (int q = 0; q < cc; ++q) back = back || cr[q] == i;
+      if (!back) asym = true;
+    }
+  });
+  const bool sym = !asym;
+  // Otherwise, the transpose: the rows j != i that hold column i, ascending.
+  std::vector<int64_t> ts;
+  std::vector<int> tj;
+  if (!sym) {
+    ts.assign(n + 1, 0);
+    for (int64_t j = 0; j < n; ++j) {
+      const int cnt = rows.nnz(j);
+      const int *rc = rows.cols(j);
+      for (int k = 0; k < cnt; ++k)
+        if (rc[k] != j) ts[rc[k] + 1]++;
+    }
+    for (int64_t i = 0; i < n; ++i) ts[i + 1] += ts[i];
+    tj.resize(std::max<int64_t>(ts[n], 1));
     std::vector<int64_t> fill(ts.begin(), ts.end() - 1);
     for (int64_t j = 0; j < n; ++j) {
       const int cnt = rows.nnz(j);
@@ -471,65 +494,97 @@ static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std:
         if (rc[k] != j) tj[fill[rc[k]]++] = (int)j;
     }
   }
-  // Breadth-first order: components in order of their smallest row s.  A far
-  // row to start from: a pass from x = s ends on row fx; while a pass from fx
-  // ends on a deeper level than the pass from x did, x moves to fx (at most 8
-  // moves).  The pass from the final fx gives the order (on a renumbered
-  // 27-point stencil it starts at a corner, and the first fit below needs
-  // the minimum, 8 colours, where natural order needs 17).
+  // The passes below walk rows in a queue whose later rows are known: the
+  // row 16 places ahead is fetched early, and the per-row flags of the row 8
+  // places ahead's columns (the accesses are random under any numbering).
+  auto ahead = [&](const std::vector<int> &q, size_t h, const auto *flag) {
+    if (h + 16 < q.size()) __builtin_prefetch(rows.cols(q[h + 16]));
+    if (h + 8 < q.size()) {
+      const int r = q[h + 8];
+      const int cnt = rows.nnz(r);
+      const int *rc = rows.cols(r);
+      for (int k = 0; k < cnt; ++k) __builtin_prefetch(flag + rc[k]);
+    }
+  };
+  // Breadth-first order: components in order of their smallest row s, each
+  // from a pseudo-peripheral row (George and Liu, ACM TOMS 5(3), 1979): a
+  // pass from x = s gives its last level and y, the row of that level with
+  // the fewest couplings; while a pass from y ends deeper than the pass from
+  // x, x moves to y (at most 8 moves); at the end x gives way to y when y has
+  // fewer couplings.  On a renumbered 27-point stencil the start is a corner
+  // of the grid, and the first fit below needs the minimum, 8 colours, where
+  // natural order needs 17.
   // Neighbours of row i: its entries in row order (diagonal skipped), then the
   // rows holding column i in ascending order.
   order.clear();
   order.reserve(n);
   {
     std::vector<char> seen(n, 0);
-    std::vector<int> probe;
-    probe.reserve(1024);
-    std::vector<int64_t> stamp(n, -1);
+    std::vector<int> stamp(n, -1);
     std::vector<int> level(n, 0);
-    int64_t stamp_id = 0;
+    int stamp_id = 0;
     auto visit = [&](int i, auto &&take) {
       const int cnt = rows.nnz(i);
       const int *rc = rows.cols(i);
       for (int k = 0; k < cnt; ++k)
         if (rc[k] != i) take(rc[k]);
-      for (int64_t t = ts[i]; t < ts[i + 1]; ++t) take(tj[t]);
+      if (!sym)
+        for (int64_t t = ts[i]; t < ts[i + 1]; ++t) take(tj[t]);
     };
-    // a breadth-first pass over s's component: its last row and that row's level
-    auto probe_pass = [&](int s, int &last, int &last_level) {
-      const int64_t id = stamp_id++;
-      probe.clear();
-      probe.push_back(s);
+    // couplings of row i in both directions (a symmetric one counts twice)
+    auto couplings = [&](int i) {
+      const int cnt = rows.nnz(i);
+      const int *rc = rows.cols(i);
+      int64_t d = 0;
+      for (int k = 0; k < cnt; ++k) d += rc[k] != i;
+      return sym ? 2 * d : d + ts[i + 1] - ts[i];
+    };
+    // a breadth-first pass over s's component into q: the depth of its last
+    // level and the row of that level with the fewest couplings (first
+    // reached on ties).  The pass from the start row is the order itself.
+    auto probe_pass = [&](int s, std::vector<int> &q, int &far, int &depth) {
+      const int id = stamp_id++;
+      q.clear();
+      q.push_back(s);
       stamp[s] = id;
       level[s] = 0;
-      for (size_t h = 0; h < probe.size(); ++h) {
-        const int i = probe[h];
+      for (size_t h = 0; h < q.size(); ++h) {
+        const int i = q[h];
+        ahead(q, h, stamp.data());
         visit(i, [&](int j) {
-          if (stamp[j] != id) { stamp[j] = id; level[j] = level[i] + 1; probe.push_back(j); }
+          if (stamp[j] != id) { stamp[j] = id; level[j] = level[i] + 1; q.push_back(j); }
         });
       }
-      last = probe.back();
-      last_level = level[last];
+      depth = level[q.back()];
+      size_t u = q.size() - 1;
+      while (u > 0 && level[q[u - 1]] == depth) --u;
+      far = q[u];
+      int64_t best = couplings(far);
+      for (++u; u < q.size(); ++u) {
+        const int64_t d = couplings(q[u]);
+        if (d < best) { best = d; far = q[u]; }
+      }
     };
+    std::vector<int> qx, qy;  // the passes from x and from its candidate
+    qx.reserve(1024);
+    qy.reserve(1024);
     for (int64_t s0 = 0; s0 < n; ++s0) {
       if (seen[s0]) continue;
-      // a far row: from x to its last row fx, moving on while that deepens
-      int fx, ex, fy, ey;
-      probe_pass((int)s0, fx, ex);
+      int x = (int)s0, fx, ex, fy, ey;
+      probe_pass(x, qx, fx, ex);
       for (int it = 0; it < 8; ++it) {
-        probe_pass(fx, fy, ey);
-        if (ey <= ex) break;
+        probe_pass(fx, qy, fy, ey);
+        if (ey <= ex) {
+          if (couplings(fx) < couplings(x)) { x = fx; qx.swap(qy); }
+          break;
+        }
+        x = fx;
+        qx.swap(qy);
         ex = ey;
         fx = fy;
       }
-      const int start = fx;
-      size_t head = order.size();
-      order.push_back(start);
-      seen[start] = 1;
-      while (head < order.size())
-        visit(order[head++], [&](int j) {
-          if (!seen[j]) { seen[j] = 1; order.push_back(j); }
-        });
+      for (const int r : qx) seen[r] = 1;
+      order.insert(order.end(), qx.begin(), qx.end());
     }
   }
   // First fit in that order: row i takes the smallest colour no coloured
@@ -538,13 +593,16 @@ static bool colour_general(RowAcc rows, int64_t n, std::vector<int> &order, std:
   int64_t mark[kMaxColours + 1];
   for (auto &m : mark) m = -1;
   ncol = 0;
-  for (const int i : order) {
+  for (size_t h = 0; h < order.size(); ++h) {
+    const int i = order[h];
+    ahead(order, h, colour.data());
     const int cnt = rows.nnz(i);
     const int *rc = rows.cols(i);
     for (int k = 0; k < cnt; ++k)
       if (rc[k] != i && colour[rc[k]] >= 0) mark[colour[rc[k]]] = i;
-    for (int64_t t = ts[i]; t < ts[i + 1]; ++t)
-      if (colour[tj[t]] >= 0) mark[colour[tj[t]]] = i;
+    if (!sym)
+      for (int64_t t = ts[i]; t < ts[i + 1]; ++t)
+        if (colour[tj[t]] >= 0) mark[colour[tj[t]]] = i;
     int pick = 0;
     while (mark[pick] == i) ++pick;
     if (pick >= kMaxColours) return false;

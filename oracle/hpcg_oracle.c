@@ -226,10 +226,20 @@ void orc_colour_greedy(const orc_level *L, int *colour, int *ncolours) {
   *ncolours = total;
 }
 
+/* Couplings of row i in both directions: its off-diagonal entries plus the
+ * rows that hold column i (a symmetric coupling counts twice). */
+static int couplings(const orc_level *L, const int *tcnt, int i) {
+  int d = tcnt[i + 1] - tcnt[i];
+  for (int k = 0; k < L->nnz_row[i]; ++k)
+    if (L->cols[(int64_t)i * NSLOT + k] != i) ++d;
+  return d;
+}
+
 /* Breadth-first pass from s over its component (rows marked with a fresh
- * stamp): the last row reached and its level. */
+ * stamp): the depth of its last level, and the row of that level with the
+ * fewest couplings (the first such row reached). */
 static void probe_bfs(const orc_level *L, const int *tcnt, const int *tj, int s, int *q, int *stamp,
-                      int *level, int *stamp_id, int *last, int *last_level) {
+                      int *level, int *stamp_id, int *far, int *depth) {
   const int id = (*stamp_id)++;
   int h = 0, t = 0;
   q[t++] = s;
@@ -244,20 +254,30 @@ static void probe_bfs(const orc_level *L, const int *tcnt, const int *tj, int s,
     for (int r = tcnt[i]; r < tcnt[i + 1]; ++r)
       if (stamp[tj[r]] != id) { stamp[tj[r]] = id; level[tj[r]] = level[i] + 1; q[t++] = tj[r]; }
   }
-  *last = q[t - 1];
-  *last_level = level[q[t - 1]];
+  *depth = level[q[t - 1]];
+  int u = t - 1;
+  while (u > 0 && level[q[u - 1]] == *depth) --u;
+  int best = q[u], bd = couplings(L, tcnt, q[u]);
+  for (++u; u < t; ++u) {
+    const int d = couplings(L, tcnt, q[u]);
+    if (d < bd) { bd = d; best = q[u]; }
+  }
+  *far = best;
 }
 
 /* Colouring of a general level: first fit in breadth-first order over the
  * symmetrised pattern.  The neighbours of row i are its own entries (row
  * order, the diagonal skipped) followed by the rows j that hold column i
- * (ascending j).  Components are taken in order of their smallest row s.  A
- * far row to start from: a breadth-first pass from x = s gives its last row
- * fx; while a pass from fx ends on a deeper level than the pass from x did,
- * x moves to fx (at most 8 moves).  The pass from the final fx gives the
- * order.  On a renumbered 27-point stencil this starts at a corner and the
- * first fit needs 8 colours, where natural order needs 17.  Row i takes the smallest colour not held by an
- * already coloured neighbour.  order[] (may be NULL) receives the sequence. */
+ * (ascending j).  Components are taken in order of their smallest row s.
+ * The start row is pseudo-peripheral (George and Liu, ACM TOMS 5(3), 1979):
+ * a breadth-first pass from x = s gives its last level, and y, the row of
+ * that level with the fewest couplings; while a pass from y ends deeper than
+ * the pass from x, x moves to y (at most 8 moves); at the end x gives way to
+ * y when y has fewer couplings.  On a renumbered 27-point stencil this start
+ * is a corner of the grid and the first fit needs 8 colours, the minimum,
+ * where natural order needs 17.  Row i takes the smallest colour not held by
+ * an already coloured neighbour.  order[] (may be NULL) receives the
+ * sequence. */
 void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, int *order) {
   const int n = L->n;
   int *tcnt = (int *)calloc((size_t)n + 1, sizeof(int));
@@ -285,17 +305,20 @@ void orc_colour_general_order(const orc_level *L, int *colour, int *ncolours, in
   int len = 0;
   for (int s0 = 0; s0 < n; ++s0) {
     if (seen[s0]) continue;
-    /* probe passes (s0's component is wholly unseen): from x to its last
-     * row fx; move to fx while that raises the last row's level */
-    int fx, ex, fy, ey;
-    probe_bfs(L, tcnt, tj, s0, probe, stamp, level, &stamp_id, &fx, &ex);
+    /* probe passes (s0's component is wholly unseen) */
+    int x = s0, fx, ex, fy, ey;
+    probe_bfs(L, tcnt, tj, x, probe, stamp, level, &stamp_id, &fx, &ex);
     for (int it = 0; it < 8; ++it) {
       probe_bfs(L, tcnt, tj, fx, probe, stamp, level, &stamp_id, &fy, &ey);
-      if (ey <= ex) break;
+      if (ey <= ex) {
+        if (couplings(L, tcnt, fx) < couplings(L, tcnt, x)) x = fx;
+        break;
+      }
+      x = fx;
       ex = ey;
       fx = fy;
     }
-    const int start = fx;
+    const int start = x;
     /* the pass that gives the order */
     int head = len;
     seq[len++] = start;

@@ -33,7 +33,9 @@ def _pack(rows, n):
 
 
 def permuted_hpcg(nx, levels=4, seed=0):
-    P = po.Problem(nx, nx, nx, levels=levels)
+    """nx: the edge of a cubic grid, or its (nx, ny, nz)."""
+    dims = tuple(nx) if isinstance(nx, (tuple, list)) else (nx, nx, nx)
+    P = po.Problem(*dims, levels=levels)
     g = np.random.default_rng(seed)
     arrs = [P.level(k).arrays() for k in range(levels)]
     perms = [g.permutation(len(a[0])) for a in arrs]  # new index of old row

@@ -33,9 +33,14 @@ def test_library_colouring_rejects_bad_rows(hip):
         hip._lib.colour_rows(nr, cols)
 
 
-@pytest.mark.parametrize("g,seed", [(16, 1), (16, 5), (24, 7), (32, 11)])
+@pytest.mark.parametrize("g,seed", [(16, 1), (16, 5), (24, 7), (32, 11), (32, 0), (32, 1), (48, 0),
+                                    ((16, 32, 8), 5), ((40, 24, 16), 6), ((8, 8, 64), 7),
+                                    ((64, 16, 16), 8)])
 def test_renumbered_stencil_gets_eight_colours(hip, g, seed):
     """8 is the minimum for the 27-point stencil (a 2x2x2 cell is a clique);
-    first fit in natural order of a random numbering needs about 17."""
+    first fit in natural order of a random numbering needs about 17.  A start
+    on a face instead of a corner of the grid (the last row of a
+    breadth-first pass rather than its least coupled one) needed 11-12 for
+    several of these numberings."""
     nr, cols, _, _ = gm.permuted_hpcg(g, levels=1, seed=seed)[0]
     assert hip._lib.colour_rows(nr, cols)[2] == 8
