@@ -245,8 +245,9 @@ def main():
                 best = el2 if best is None else min(best, el2)
             ctx.spmv(0, dx, dAx)
             rel = ctx.waxpby_dot(1.0, db, -1.0, dAx, dAx) ** 0.5 / bn
+            fwd = forward_error_ones(ctx, dx, shapes[0][0])
             tol_res[name] = {"seconds": round(best, 4), "iterations": o["niters"],
-                             "true_rel_residual": float(f"{rel:.4e}")}
+                             "true_rel_residual": float(f"{rel:.4e}"), "forward_error": float(f"{fwd:.4e}")}
             log(f"[bench] {name}: {o['niters']} it, {best * 1e3:.1f} ms, true rel res {rel:.3e}")
         tol_res["speedup_fp32_vs_fp64"] = round(tol_res["fp64"]["seconds"] / tol_res["mixed_fp32"]["seconds"], 3)
         tol_res["speedup_fp16_vs_fp64"] = round(tol_res["fp64"]["seconds"] / tol_res["mixed_fp16"]["seconds"], 3)
@@ -290,6 +291,16 @@ def solve_timed(ctx, db, dx, variant, tol, reps=1, max_iter=5000):
     return out
 
 
+def forward_error_ones(ctx, dx, n):
+    """||x - 1|| / ||1||: the HPCG problem's exact solution is all ones
+    (b = A 1 exactly, GenerateProblem_ref.cpp:171-180)."""
+    e = ctx.vec(0)
+    ctx.fill(e, 1.0)
+    err2 = ctx.waxpby_dot(1.0, dx, -1.0, e, e)
+    e.free()
+    return (err2 / n) ** 0.5
+
+
 def true_relres(ctx, db, dx, bnorm):
     r = ctx.vec(0)
     ctx.spmv(0, dx, r)
@@ -327,13 +338,14 @@ def case_mixed_512(args):
                                     ("mixed_fp16", hpcg_amd.HPCGX_CG_MIXED_FP16, 2, 4)):
         o = solve_timed(ctx, db, dx, variant, 1e-9)
         rel = true_relres(ctx, db, dx, bn)
+        fwd = forward_error_ones(ctx, dx, shapes[0][0])
         per_it = o["seconds"] / max(1, o["niters"])
         by = algorithmic_bytes_per_iteration(g, g, g, v_mg=vmg, s_mg=smg)
         # HPCG-MxP-style rate: the HPCG model's FP64 operation count of the
         # iterations performed over the mixed solve's time (not an HPCG rating)
         mxp = hpcg_flops(shapes, o["niters"], sets=1)["total"] / o["seconds"] / 1e9
         res[name] = {"seconds": round(o["seconds"], 3), "iterations": o["niters"],
-                     "true_rel_residual": float(f"{rel:.4e}"), "ms_per_iteration": round(per_it * 1e3, 3),
+                     "true_rel_residual": float(f"{rel:.4e}"), "forward_error": float(f"{fwd:.4e}"), "ms_per_iteration": round(per_it * 1e3, 3),
                      "algorithmic_tb_s": round(by / per_it / 1e12, 3), "hpcg_model_gflops": round(mxp, 1)}
         log(f"[bench mixed] {name}: {o['niters']} it, {o['seconds']:.2f} s, rel res {rel:.3e}, "
             f"{per_it * 1e3:.2f} ms/it, {by / per_it / 1e12:.2f} TB/s")
