@@ -1299,9 +1299,10 @@ int hpcgx_level_set_diagonal(hpcgx_ctx *c, int level, const double *diag) {
   k_set_diag<<<nblk(L.lay.n), kBlock, 0, c->stream>>>(L.vals, L.dslot, L.perm, c->stage, L.lay.n);
   HPCGX_CHECK_LAUNCH(c);
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  // the lower-precision copies (and the fp16 scaling) follow the new diagonal
   const int prepared = c->prepared;
-  if (prepared & 1) (void)build_precision(c, HPCGX_PREC_FP32);
-  if (prepared & 2) (void)build_precision(c, HPCGX_PREC_FP16);
+  if ((prepared & 1) && (rc = build_precision(c, HPCGX_PREC_FP32))) return rc;
+  if ((prepared & 2) && (rc = build_precision(c, HPCGX_PREC_FP16))) return rc;
   return HPCGX_OK;
 }
 
