@@ -281,6 +281,8 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
 // issue back to back.
 // FINE: the finest level (streamed matrix loads; its own instantiation, so
 // profiles show the dominant launches separately from the coarse levels').
+// ZK == 0 reads no neighbour at all (x_i = r_i / a_ii), which holds for the
+// first block of any order, general levels included.
 // GEN: general level, the diagonal's slot is dslot[row] (stencil: slot 13).
 // gs_row is the update of one row; it returns r_i * x_i (new) for the fused r.z.
 template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI, bool GEN, bool SAFE>
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__rest
                                                const XT *__restrict__ r, XT *__restrict__ x,
                                                int64_t row0, int64_t nrows,
                                                const uint8_t *__restrict__ dslot, double *rz = nullptr) {
-  static_assert(!GEN || ZK < 0, "general levels sweep full rows");
+  static_assert(!GEN || ZK <= 0, "general levels: full rows, or the first block from x = 0 (diagonal only)");
   static_assert(!RZ || ZK < 0, "r.z is taken from the post-smoother");
   const int64_t t = tile_id() * kBlock + threadIdx.x;
   if constexpr (RZ) {

@@ -41,12 +41,12 @@ static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, in
       return;
     }
   }
-  if constexpr (ZK < 0) {
+  if constexpr (ZK <= 0) {  // general levels: full rows, or the first block from x = 0
     if (L.dslot) {
       if (ci.cbase)
-        k_gs<VT, XT, ACC, -1, FINE, true, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, L.dslot);
+        k_gs<VT, XT, ACC, ZK, FINE, true, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, L.dslot);
       else
-        k_gs<VT, XT, ACC, -1, FINE, false, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, L.dslot);
+        k_gs<VT, XT, ACC, ZK, FINE, false, true><<<g, kBlock, 0, st>>>(ci, v, r, x, r0, nr, L.dslot);
       return;
     }
   }
@@ -88,7 +88,9 @@ static bool default_order(const Level &L) {
 // block would recompute it from exactly the neighbour values its forward step
 // used, so it is elided.  zero_start: x is known to be zero on entry (the
 // V-cycle's first sweep); in the default stencil order x is then not read
-// before it is written, otherwise it is zero-filled first.
+// before it is written; otherwise (general levels, other orders) it is
+// zero-filled first, and block 0, whose neighbours are all still zero, reads
+// only its diagonal (x_i = r_i / a_ii, what the full step gives bit for bit).
 // rz: the steps that leave their rows final (forward step of the last block,
 // then the backward steps) write r.x partials at rz, *nrz of them in all.
 // untrusted_x: x is a caller's vector whose contents on a zero start may be
@@ -97,11 +99,11 @@ template <class VT, class XT, class ACC, bool FINE>
 static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start, double *rz = nullptr,
                   int64_t *nrz = nullptr, bool untrusted_x = false) {
   const VT *v = vals_of<VT>(L);
-  if (zero_start && !default_order(L)) {  // other orders: zero x, then full steps
+  if (zero_start && !default_order(L)) {
     k_zero<XT><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(x, L.lay.npad);
-    zero_start = false;
-  }
-  if (zero_start) {
+    gs_block<VT, XT, ACC, 0, FINE>(c, L, v, r, x, 0);
+    for (int k = 1; k < L.lay.ncol; ++k) gs_block<VT, XT, ACC, -1, FINE>(c, L, v, r, x, k);
+  } else if (zero_start) {
     if (untrusted_x) zero_start_forward<VT, XT, ACC, FINE, true>(c, L, v, r, x);
     else zero_start_forward<VT, XT, ACC, FINE, false>(c, L, v, r, x);
   } else {
