@@ -33,13 +33,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_rz(int64_t n, const double *__res
   block_partials<1>(v, partials);
 }
 
+// The preconditioned residual as the solver reads it: z_i, or for the
+// scaled fp16 V-cycle z_i = zhat_i * dinv_i (rounded in fp32, as the
+// V-cycle's own unscaling would round it).
+__device__ __forceinline__ double z_at(const float *__restrict__ z, const float *__restrict__ zscale, int64_t i) {
+  return zscale ? (double)(z[i] * zscale[i]) : (double)z[i];
+}
+__device__ __forceinline__ double z_at(const double *__restrict__ z, const float *, int64_t i) { return z[i]; }
+
 // p = z + beta p  (first: z + 0.0 z, the WAXPBY(1,z,0,z) of CG.cpp:109)
 template <class ZT>
 __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict__ z, double *__restrict__ p,
-                                                 const double *__restrict__ scal, int first) {
+                                                 const double *__restrict__ scal, int first,
+                                                 const float *__restrict__ zscale = nullptr) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const double zi = (double)z[i];
+  const double zi = z_at(z, zscale, i);
   p[i] = first ? zi + 0.0 * zi : zi + scal[S_BETA] * p[i];
 }
 
@@ -78,14 +87,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const ColIdx ci, const d
 }
 
 // x += alpha p ; r -= alpha Ap ; normr2 = r.r
-// R32: also leave r in fp32 for the next mixed-precision V-cycle.
+// R32: also leave r in fp32 for the next mixed-precision V-cycle, times
+// rscale[i] when given (the scaled fp16 V-cycle's input dinv_i r_i, rounded
+// as (float)r_i * dinv_i).
 template <bool R32>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restrict__ x,
                                                       const double *__restrict__ p, double *__restrict__ r,
                                                       const double *__restrict__ Ap, float *__restrict__ r32,
-                                                      double *partials, const double *__restrict__ scal) {
+                                                      double *partials, const double *__restrict__ scal,
+                                                      const float *__restrict__ rscale = nullptr) {
   const double alpha = scal[S_ALPHA], malpha = -alpha;
   double xv[kChunk], pv[kChunk], rv[kChunk], qv[kChunk];
+  float sv[kChunk];
 #pragma unroll
   for (int j = 0; j < kChunk; ++j) {
     const int64_t i = chunk_index(j);
@@ -94,6 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
     pv[j] = in ? p[i] : 0.0;
     rv[j] = in ? r[i] : 0.0;
     qv[j] = in ? Ap[i] : 0.0;
+    sv[j] = (R32 && in && rscale) ? rscale[i] : 1.0f;
   }
   double acc = 0.0;
 #pragma unroll
@@ -103,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
     if (i < n) {
       x[i] = xv[j] + alpha * pv[j];
       r[i] = ri;
-      if (R32) r32[i] = (float)ri;
+      if (R32) r32[i] = rscale ? (float)ri * sv[j] : (float)ri;
       acc += ri * ri;
     }
   }
@@ -112,16 +126,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
 }
 
 // r = b - Ap ; normr2 = r.r (CG.cpp:88-89) ; optional fp32 copy of r
+// (times rscale[i] when given, as in k_cg_update)
 template <bool R32>
 __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double *__restrict__ b,
                                                         const double *__restrict__ Ap, double *__restrict__ r,
-                                                        float *__restrict__ r32, double *partials) {
+                                                        float *__restrict__ r32, double *partials,
+                                                        const float *__restrict__ rscale = nullptr) {
   double bv[kChunk], qv[kChunk];
+  float sv[kChunk];
 #pragma unroll
   for (int j = 0; j < kChunk; ++j) {
     const int64_t i = chunk_index(j);
     bv[j] = i < n ? b[i] : 0.0;
     qv[j] = i < n ? Ap[i] : 0.0;
+    sv[j] = (R32 && i < n && rscale) ? rscale[i] : 1.0f;
   }
   double acc = 0.0;
 #pragma unroll
@@ -130,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double 
     const double ri = bv[j] + -1.0 * qv[j];
     if (i < n) {
       r[i] = ri;
-      if (R32) r32[i] = (float)ri;
+      if (R32) r32[i] = rscale ? (float)ri * sv[j] : (float)ri;
       acc += ri * ri;
     }
   }
@@ -142,14 +160,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double 
 // beta = -alpha_{k-1} zAp / oldrtz, since r_k - r_{k-1} = -alpha_{k-1} A p_{k-1}.
 __global__ __launch_bounds__(kBlock) void k_fcg_rz(int64_t n, const double *__restrict__ r,
                                                    const float *__restrict__ z, const double *__restrict__ Ap,
-                                                   double *partials, int restart) {
+                                                   double *partials, int restart,
+                                                   const float *__restrict__ zscale) {
   double rv[kChunk], zv[kChunk], qv[kChunk];
 #pragma unroll
   for (int j = 0; j < kChunk; ++j) {
     const int64_t i = chunk_index(j);
     const bool in = i < n;
     rv[j] = in ? r[i] : 0.0;
-    zv[j] = in ? (double)z[i] : 0.0;
+    zv[j] = in ? z_at(z, zscale, i) : 0.0;
     qv[j] = (in && !restart) ? Ap[i] : 0.0;
   }
   double a0 = 0.0, a1 = 0.0;
@@ -162,10 +181,11 @@ __global__ __launch_bounds__(kBlock) void k_fcg_rz(int64_t n, const double *__re
   block_partials<2>(v, partials);
 }
 
-// x[...] copy for p = z on restart (fp32 z)
-__global__ __launch_bounds__(kBlock) void k_copy_z(int64_t n, const float *__restrict__ z, double *__restrict__ p) {
+// p = z on restart (fp32 z)
+__global__ __launch_bounds__(kBlock) void k_copy_z(int64_t n, const float *__restrict__ z, double *__restrict__ p,
+                                                   const float *__restrict__ zscale) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = (double)z[i];
+  if (i < n) p[i] = z_at(z, zscale, i);
 }
 
 // dL/dkappa (see hpcgx_kappa_grad); neighbours in ascending slot order
@@ -303,13 +323,18 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
   const int64_t np = L0.lay.npad;
   const unsigned G = chunk_blocks(np), T = tiles(np);
   double *r = c->cg_r, *p = c->cg_p, *Ap = c->cg_Ap;
-  float *r32 = c->cg_r32, *z32 = c->cg_z32;
+  // The V-cycle's input and output.  Scaled fp16: the residual updates write
+  // dinv r straight into the level's fp16-cycle input, and the kernels that
+  // read z apply z = dinv zhat as they load it (no scaling passes).
+  const bool f16 = variant == HPCGX_CG_MIXED_FP16;
+  float *r32 = f16 ? L0.r16 : c->cg_r32, *z32 = c->cg_z32;
+  const float *dscale = f16 ? L0.dinv : nullptr;
   int rc;
   PhaseTimer tm{c, times != nullptr};
   auto true_residual = [&](double *n2) -> int {
     int e = spmv_fp64(c, 0, x, Ap);
     if (e) return e;
-    k_cg_residual<true><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, r32, c->partials);
+    k_cg_residual<true><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, r32, c->partials, dscale);
     HPCGX_CHECK_LAUNCH(c);
     if ((e = finish(c, 1, G, F_NORMR2))) return e;
     return read_scalar(c, S_NORMR2, n2);
@@ -331,19 +356,16 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
     }
     ++k;
     tm.mark(0);
-    if (variant == HPCGX_CG_MIXED_FP16)
-      rc = mg_fp16(c, r32, z32);
-    else
-      rc = mg_fp32(c, 0, r32, z32);
+    rc = f16 ? mg_fp16_scaled(c, r32, z32) : mg_fp32(c, 0, r32, z32);
     if (rc) return rc;
     tm.mark(1);
-    k_fcg_rz<<<G, kBlock, 0, c->stream>>>(np, r, z32, Ap, c->partials, restart ? 1 : 0);
+    k_fcg_rz<<<G, kBlock, 0, c->stream>>>(np, r, z32, Ap, c->partials, restart ? 1 : 0, dscale);
     finish(c, 2, G, restart ? F_RZ_FIRST : F_FCG);
     tm.mark(2);
     if (restart)
-      k_copy_z<<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p);
+      k_copy_z<<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, dscale);
     else
-      k_cg_p<float><<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, c->scal, 0);
+      k_cg_p<float><<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, c->scal, 0, dscale);
     restart = false;
     tm.mark(3);
     if (L0.cbase)
@@ -354,7 +376,7 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
                                                         np, c->partials);
     finish(c, 1, T, F_PAP);
     tm.mark(4);
-    k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal);
+    k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal, dscale);
     finish(c, 1, G, F_NORMR2);
     tm.mark(5);
     HPCGX_CHECK_LAUNCH(c);

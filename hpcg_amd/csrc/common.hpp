@@ -216,6 +216,7 @@ int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz = nullptr,
             bool untrusted_x = false);
 int mg_fp32(Ctx *c, int level, const float *r, float *x);
 int mg_fp16(Ctx *c, const float *r, float *x);
+int mg_fp16_scaled(Ctx *c, const float *rhat, float *xhat);
 int spmv_fp64(Ctx *c, int level, const double *x, double *y);
 int symgs_fp64(Ctx *c, int level, const double *r, double *x);
 int ensure_reduce(Ctx *c, int64_t grid);

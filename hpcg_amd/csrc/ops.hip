@@ -238,6 +238,12 @@ __global__ __launch_bounds__(kBlock) void k_scale(const float *__restrict__ a, c
   if (i < n) o[i] = a[i] * s[i];
 }
 
+// The scaled fp16 V-cycle proper: xhat from rhat = dinv r (a solver that
+// applies the scalings itself).
+int mg_fp16_scaled(Ctx *c, const float *rhat, float *xhat) {
+  return vcycle<__half, float, float, true>(c, 0, rhat, xhat);
+}
+
 // Scaled fp16 V-cycle on level 0: rhat = dinv r, V-cycle with D^-1/2 A D^-1/2,
 // x = dinv xhat.
 int mg_fp16(Ctx *c, const float *r, float *x) {

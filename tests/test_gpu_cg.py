@@ -166,6 +166,19 @@ def test_mixed_cg_converges(hip, variant):
     assert abs(out["niters"] - want["niters"]) <= 1
     fp64 = P.cg(b, max_iter=200, tol=1e-9, mode=2)
     assert out["niters"] <= fp64["niters"] + 3
+    # The residual history of 30 iterations follows the oracle's flexible
+    # PCG.  The V-cycles are bit-exact and only the dots' summation order
+    # differs (~1e-16), but r is rounded to fp32 on its way into the V-cycle,
+    # where such a difference can flip an element's rounding: the histories
+    # then part at the fp32 level (measured 1.5e-6 relative for both
+    # precisions).  A lost or doubled scaling would part them at O(1).
+    ctx.fill(dx, 0.0)
+    got = ctx.cg(db, dx, max_iter=30, tol=0.0, variant=cgv, history=True)
+    hw = P.cg_mixed(b, max_iter=30, tol=0.0, mg_prec=32 if variant == "fp32" else 16, history=True)
+    h, hr = got["hist"] / got["normr0"], hw["hist"] / hw["normr0"]
+    live = hr > 1e-10
+    assert got["niters"] == hw["niters"] == 30
+    assert np.allclose(h[live], hr[live], rtol=1e-5, atol=0), np.max(np.abs(h[live] / hr[live] - 1))
 
 
 def _kappa(n, seed=7):
