@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict
 // CI: column indices from the compact form (the mixed-precision solvers,
 // which already stand outside the reference's rules; values stay fp64).
 template <bool CI>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(const ColIdx ci, const double *__restrict__ vals,
+__global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(double, CI) void k_cg_spmv_dot(const ColIdx ci, const double *__restrict__ vals,
                                                         const double *__restrict__ p,
                                                         double *__restrict__ Ap, int64_t nrows,
                                                         double *partials) {

@@ -30,6 +30,21 @@ __device__ __forceinline__ int64_t tile_id() {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Occupancy of the fp64 sparse kernels with explicit column indices (SpMV,
+// Gauss-Seidel steps, restriction): at most two waves per SIMD, so a wave
+// holds the column, value and gather loads of all its slots in flight (169
+// VGPRs) instead of the default's batches of about 9 at six waves per SIMD.
+// Measured on the HPCG sets: 3 % faster at 512^3 (the CG SpMV 9 %), 1.7 % at
+// 192^3; eight waves per SIMD (64 VGPRs) is 10 % slower.  The fp32 / fp16 and
+// compact-index kernels keep the default (slower with the limit), and so do
+// steps that read fewer than 8 slots (the backend pads every kernel under the
+// limit to its register budget).
+template <class VT, bool CI = false, int SLOTS = 27>
+struct SparseOccupancy {
+  static constexpr unsigned max_waves = (sizeof(VT) == 8 && !CI && SLOTS >= 8) ? 2 : 8;
+};
+#define HPCGX_SPARSE_OCCUPANCY(VT, CI) __attribute__((amdgpu_waves_per_eu(1, SparseOccupancy<VT, CI>::max_waves)))
+
 // Matrix loads.  NT (non-temporal): the fine level's matrix is streamed once
 // per pass and is far larger than any cache, so it is kept out of the way of
 // the vectors; coarse-level matrices fit in the 256 MB Infinity Cache and
@@ -242,7 +257,7 @@ __device__ __forceinline__ int64_t chunk_index(int j) {
 
 // y = A x over rows [row0, row0 + nrows); ComputeSPMV_ref.cpp:61-70
 template <class VT, class XT, class ACC>
-__global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ cols,
+__global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, false) void k_spmv(const int *__restrict__ cols,
                                                  const VT *__restrict__ vals,
                                                  const XT *__restrict__ x, XT *__restrict__ y,
                                                  int64_t nrows) {
@@ -333,7 +348,9 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
 // r.z needs no pass of its own.
 template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI = false, bool GEN = false, bool RZ = false,
           bool SAFE = false>
-__global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__restrict__ vals,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(1, SparseOccupancy<VT, CI, ZK < 0 ? kSlots : __builtin_popcount(swept_before_mask(ZK))>::max_waves)))
+void k_gs(const ColIdx ci, const VT *__restrict__ vals,
                                                const XT *__restrict__ r, XT *__restrict__ x,
                                                int64_t row0, int64_t nrows,
                                                const uint8_t *__restrict__ dslot, double *rz = nullptr) {
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_gs(const ColIdx ci, const VT *__rest
 // injected row f = row0 + t (frow[t] when frow is given).  (A x)_f is summed
 // from 0 like SpMV, then subtracted.  SCALE: fp16 V-cycle, times sigma[t].
 template <class VT, class XT, class ACC, bool SCALE, bool CI = false>
-__global__ __launch_bounds__(kBlock) void k_restrict(const ColIdx ci, const VT *__restrict__ vals,
+__global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_restrict(const ColIdx ci, const VT *__restrict__ vals,
                                                      const XT *__restrict__ r,
                                                      const XT *__restrict__ x,
                                                      XT *__restrict__ rc, const int *__restrict__ inj,
