@@ -124,7 +124,7 @@ def main():
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-1e-9 solves")
-    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--case", default="hpcg", choices=("hpcg", "mixed", "kappa", "adjoint", "general"))
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="first copy 1 GiB with a plain copy kernel (scale for FETCH_SIZE/WRITE_SIZE)")
