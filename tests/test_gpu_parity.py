@@ -13,9 +13,12 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
-# (16, 256, 8) and (32, 256, 16) have blocks with >= 128 rows of Y: their
-# finest rows are stored in Y strips (block_index), the others in natural order.
-GRIDS = [(16, 16, 16), (24, 24, 24), (32, 16, 24), (16, 256, 8), (32, 256, 16)]
+# (16, 256, 8), (32, 256, 16) and (256, 256, 8) have blocks with >= 128 rows
+# of Y: their finest rows are stored in Y strips (block_index; strips of 16
+# rows of Y for the last), the others in natural order.  (8, 8, 8) is the
+# smallest grid with 4 levels (the coarsest is 1 x 1 x 1).
+GRIDS = [(8, 8, 8), (16, 16, 16), (24, 24, 24), (32, 16, 24), (64, 48, 40), (16, 256, 8), (32, 256, 16),
+         (256, 256, 8)]
 
 
 @pytest.fixture(scope="module")
