@@ -24,6 +24,10 @@ CASES = {
     # upload without a grid
     "random_spd_rows": (lambda: gm.random_spd(seed=2), None),
     "unsymmetric_rows": (lambda: gm.unsymmetric(seed=3), None),
+    # larger and non-cubic renumbered hierarchies (more slices per colour,
+    # list transfers over thousands of injected rows)
+    "permuted_hpcg_32_rows": (lambda: gm.permuted_hpcg(32, seed=5), None),
+    "permuted_hpcg_48x32x16_grid": (lambda: gm.permuted_hpcg((48, 32, 16), seed=6), (48, 32, 16)),
 }
 
 
