@@ -110,9 +110,9 @@ def cpu_baseline(g, iters):
     fl = hpcg_flops(level_shapes(g, g, g), out["niters"], sets=1)["total"]
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(fl / el / 1e9, 4), "unit": "GFLOP/s", "cores": cores, "kind": "reference",
-            "sample": f"CG_ref, {out['niters']} iterations + preamble at {g}^3 after setup "
-                      f"({el:.1f} s), OpenMP build with setup/Make.GCC_OMP flags; its Gauss-Seidel "
-                      f"is serial (ComputeSYMGS_ref.cpp has no OpenMP)"}
+            "sample": f"CG_ref, {out['niters']} iterations + preamble at {g}^3, timed after the problem "
+                      f"setup: {el:.1f} s of CG; OpenMP build with setup/Make.GCC_OMP flags; its "
+                      f"Gauss-Seidel is serial (ComputeSYMGS_ref.cpp has no OpenMP)"}
 
 
 def main():
