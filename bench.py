@@ -20,8 +20,8 @@ Other workloads print their own JSON line:
     --case adjoint   128^3 adjoint through torch.autograd: backward/forward time,
                      finite differences on 64 kappa entries, fp64 CPU gradient
     --case general   128^3 HPCG hierarchy renumbered by a random permutation per
-                     level (no grid describes it): the general greedy-coloured
-                     layout against the stencil layout of the same matrix
+                     level (no grid describes it): the general layout (first-fit
+                     colours in breadth-first order) against the stencil layout
 """
 import argparse
 import json

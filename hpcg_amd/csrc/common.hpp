@@ -51,15 +51,16 @@ constexpr int kDefaultOrder[kColours] = {7, 3, 4, 5, 2, 6, 1, 0};
 // block k holding the rows of parity class cls[k]; inside a block rows keep
 // natural order of (x>>1, y>>1, z>>1), and slot s of a row is the stencil
 // offset s.  General layout (rows that are not a 27-point stencil of their
-// grid): ncol blocks from a greedy colouring, rows ascending inside a block,
-// and slot j of a row is the row's j-th entry.  Each block is padded to a
+// grid): ncol blocks from a first-fit colouring in breadth-first order, rows
+// in that breadth-first order inside a block, and slot j of a row is the
+// row's j-th entry.  Each block is padded to a
 // multiple of 64 rows; padded rows hold the identity row (diagonal 1 in the
 // diagonal slot, other slots 0) and zero vectors.
 struct Layout {
   int nx = 0, ny = 0, nz = 0;
   int64_t n = 0, npad = 0;
   int ncol = kColours;                   // colour blocks
-  bool general = false;                  // greedy colouring of arbitrary rows
+  bool general = false;                  // first-fit colouring of arbitrary rows
   int64_t start[kMaxColours + 1] = {};   // padded block starts
   int64_t count[kMaxColours] = {};       // true rows per block
   // stencil layout only

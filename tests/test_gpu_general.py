@@ -4,8 +4,8 @@ tests/test_oracle_general.py).
 
 The reference requires every optimisation to work for general sparse
 matrices (GenerateProblem.cpp:47-49).  Rows that are not the 27-point stencil
-of their grid take the general layout: greedy colouring, rows' own entry
-order, the diagonal wherever the row holds it, and transfers over a list of
+of their grid take the general layout: first-fit colouring in breadth-first
+order, rows' own entry order, the diagonal wherever the row holds it, and transfers over a list of
 injected rows.  Bars as for the stencil layout: bit-exact matrix, colouring,
 SpMV, symmetric Gauss-Seidel, restriction, prolongation and V-cycles (fp64,
 fp32, scaled fp16); the CG residual history within 1e-8 relative.
