@@ -1,9 +1,10 @@
 """Seeded sweep over grid shapes: 20 grids with random extents (1 to 40 per
 dimension, divisible as the level count needs) and 1 to 4 levels, each
 generated on the device and checked against the oracle: SpMV, the coloured
-symmetric Gauss-Seidel and the V-cycle bit for bit, the CG residual history
-within 1e-8 relative.  Extents that are odd, 1 or 2 exercise empty and
-single-row colour blocks, padded slices and small coarse levels."""
+symmetric Gauss-Seidel and the V-cycle (fp64, fp32, scaled fp16) bit for
+bit, the CG residual history within 1e-8 relative.  Extents that are odd, 1
+or 2 exercise empty and single-row colour blocks, padded slices and small
+coarse levels."""
 import numpy as np
 import pytest
 
@@ -47,4 +48,11 @@ def test_random_grid_matches_oracle(hip, dims, levels):
     h, hw = got["hist"] / got["normr0"], want["hist"] / want["normr0"]
     live = hw > 1e-10
     assert np.allclose(h[live], hw[live], rtol=1e-8, atol=0)
+    # the fp32 and scaled fp16 V-cycles, bit for bit
+    r32 = r.astype(np.float32)
+    for code, want_mg in ((hip.HPCGX_PREC_FP32, P.mg_f32), (hip.HPCGX_PREC_FP16, P.mg_f16)):
+        ctx.prepare(code)
+        dm = ctx.vec(0)
+        ctx.mg_mixed(code, dr, dm)
+        assert np.array_equal(dm.download(), want_mg(r32).astype(np.float64)), f"mg precision {code}"
     ctx.close()
