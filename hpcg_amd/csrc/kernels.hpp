@@ -356,7 +356,9 @@ void k_gs(const ColIdx ci, const VT *__restrict__ vals,
                                                const uint8_t *__restrict__ dslot, double *rz = nullptr) {
   static_assert(!GEN || ZK <= 0, "general levels: full rows, or the first block from x = 0 (diagonal only)");
   static_assert(!RZ || ZK < 0, "r.z is taken from the post-smoother");
-  const int64_t t = tile_id() * kBlock + threadIdx.x;
+  // blocks of 256 threads, or of 64 for the small coarse fp64 steps (one wave
+  // per block spreads them over more CUs); the r.z form always takes 256
+  const int64_t t = tile_id() * (int64_t)blockDim.x + threadIdx.x;
   if constexpr (RZ) {
     // every thread reaches the block reduction; rows past the block add 0
     double acc = 0.0;
