@@ -32,6 +32,7 @@ enum Scalar : int {
   S_BETA,
   S_DOT,
   S_DOT2,
+  S_STOP,  // deferred stop test of a tolerance-0 solve: nonzero once a norm was not > 0
   S_COUNT = 16
 };
 
@@ -170,8 +171,9 @@ struct Ctx {
   double *folded = nullptr;    // second-level partials of large reductions (same allocation)
   int64_t reduce_cap = 0;      // largest grid the scratch covers
   double *scal = nullptr;      // device scalars [S_COUNT]
-  double *hist = nullptr;      // device residual history
+  double *hist = nullptr;      // device norms of a deferred-stop batch
   int hist_cap = 0;
+  std::vector<hipEvent_t> evpool;  // phase events of a deferred-stop batch (6 per iteration)
   double *hbuf = nullptr;      // pinned host scratch
   // solver buffers (level-0 length)
   double *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
@@ -212,16 +214,19 @@ int ensure_stage(Ctx *c, int64_t n);
 
 // launch helpers (implemented in ops.hip)
 // rz: if given, the partials of r.x (result) are left there, *nrz of them.
-// untrusted_x: x is a caller's vector (its old contents must not leak in).
+// untrusted_x: x may hold anything (its old contents must not leak in).
 int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz = nullptr, int64_t *nrz = nullptr,
             bool untrusted_x = false);
-int mg_fp32(Ctx *c, int level, const float *r, float *x);
-int mg_fp16(Ctx *c, const float *r, float *x);
+int mg_fp32(Ctx *c, int level, const float *r, float *x, bool untrusted_x = false);
+int mg_fp16(Ctx *c, const float *r, float *x, bool untrusted_x = false);
 int mg_fp16_scaled(Ctx *c, const float *rhat, float *xhat);
 int spmv_fp64(Ctx *c, int level, const double *x, double *y);
 int symgs_fp64(Ctx *c, int level, const double *r, double *x);
+int zero_vec(Ctx *c, double *x, int64_t n);
+int zero_vec(Ctx *c, float *x, int64_t n);
 int ensure_reduce(Ctx *c, int64_t grid);
-int finish(Ctx *c, int nv, int64_t grid, int op);
+// slot (F_NORMR2 only): record the norm there and run the deferred stop test
+int finish(Ctx *c, int nv, int64_t grid, int op, double *slot = nullptr);
 inline unsigned chunk_blocks(int64_t n) { return (unsigned)((n + 4 * 256 - 1) / (4 * 256)); }
 
 }  // namespace hpcgx

@@ -160,6 +160,7 @@ enum FinishOp : int {
   F_RZ,             // oldrtz = rtz; rtz = t0; beta = rtz/oldrtz  (CG.cpp:112-114)
   F_PAP,            // pAp = t0; alpha = rtz/pAp                  (CG.cpp:119-120)
   F_NORMR2,         // normr^2 = t0                               (CG.cpp:123)
+                    // with a slot: record t0 there; stop test t0 > 0
   F_FCG,            // oldrtz = rtz; rtz = t0; beta = -alpha t1 / oldrtz
 };
 
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_fold(const double *__restrict__ in, 
 
 template <int NV>
 __global__ __launch_bounds__(kFinishThreads) void k_finish(const double *__restrict__ partials, int64_t G,
-                                                           double *scal, int op) {
+                                                           double *scal, int op, double *slot) {
   __shared__ double sh[kFinishThreads / 64][NV];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -234,7 +235,21 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const double *__restr
       scal[S_PAP] = t[0];
       scal[S_ALPHA] = scal[S_RTZ] / t[0];
       break;
-    case F_NORMR2: scal[S_NORMR2] = t[0]; break;
+    case F_NORMR2:
+      // Deferred stop test of a tolerance-0 solve (CG.cpp:100 with tolerance
+      // 0): with normr0 finite and positive, normr/normr0 > 0 holds exactly
+      // when normr^2 > 0.  The first norm that fails (zero or NaN) raises
+      // S_STOP; from then on k_cg_update leaves x and r alone and no norm
+      // is recorded.  Without a slot the flag is cleared (solver preamble).
+      if (slot) {
+        if (scal[S_STOP] != 0.0) break;
+        *slot = t[0];
+        if (!(t[0] > 0.0)) scal[S_STOP] = 1.0;
+      } else {
+        scal[S_STOP] = 0.0;
+      }
+      scal[S_NORMR2] = t[0];
+      break;
     case F_FCG: {
       const double old = scal[S_RTZ];
       scal[S_OLDRTZ] = old;
@@ -319,11 +334,11 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
       const int c = col_at<CM, FINE>(ci, row, slice, b, s);
       const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
       const ACC xc = (ACC)x[c];
-      // SAFE (from x = 0 into a caller's vector): a zero entry (a neighbour
+      // SAFE (from x = 0 into a vector that may hold anything): a zero entry (a neighbour
       // outside the grid) may point at a row not written yet in this sweep,
       // whose old value is not trusted: it contributes 0 * 0, as the
-      // reference's zeroed x gives.  The solvers' own vectors always hold
-      // finite values, so their sweeps skip the select.
+      // reference's zeroed x gives.  The solvers clear their level-0
+      // buffer when a solve begins, so their level-0 sweeps skip the select.
       if constexpr (ZK >= 0 && SAFE) sum -= v * (v != (ACC)0 ? xc : (ACC)0);
       else sum -= v * xc;
     }

@@ -100,8 +100,9 @@ static bool default_order(const Level &L) {
 // only its diagonal (x_i = r_i / a_ii, what the full step gives bit for bit).
 // rz: the steps that leave their rows final (forward step of the last block,
 // then the backward steps) write r.x partials at rz, *nrz of them in all.
-// untrusted_x: x is a caller's vector whose contents on a zero start may be
-// anything (not the solvers' own buffers, which always hold finite values).
+// untrusted_x: x may hold anything on a zero start (a caller's vector, or a
+// coarse level's vector after a V-cycle on a non-finite input); the solvers
+// clear their own level-0 buffer when a solve begins and skip the guard.
 template <class VT, class XT, class ACC, bool FINE>
 static void sweep(Ctx *c, const Level &L, const XT *r, XT *x, bool zero_start, double *rz = nullptr,
                   int64_t *nrz = nullptr, bool untrusted_x = false) {
@@ -142,8 +143,9 @@ static int symgs_launch(Ctx *c, int l, const XT *r, XT *x, bool zero_start, doub
 // V-cycle of ComputeMG_ref.cpp:39-63: x = 0; pre-smooth; residual + restrict;
 // recurse; prolong; post-smooth.  The coarsest level gets one smoothing step.
 // rz / nrz: r.x partials of the result from the post-smoother (see sweep);
-// fp64 stencil levels with a coarse level only.  untrusted_x: x is the
-// caller's (the coarse levels' vectors are the context's own).
+// fp64 stencil levels with a coarse level only.  untrusted_x: x at level l
+// may hold anything (the caller's vector); the coarse levels always take
+// this form.
 template <class VT, class XT, class ACC, bool SCALE>
 static int vcycle(Ctx *c, int l, const XT *r, XT *x, double *rz = nullptr, int64_t *nrz = nullptr,
                   bool untrusted_x = false) {
@@ -168,7 +170,9 @@ static int vcycle(Ctx *c, int l, const XT *r, XT *x, double *rz = nullptr, int64
     k_restrict<VT, XT, ACC, SCALE, false><<<tiles(nc), kBlock, 0, c->stream>>>(ci, vals_of<VT>(L), r, x, rc_v,
                                                                               L.inj, L.sigma, r0, nc, L.frow);
   HPCGX_CHECK_LAUNCH(c);
-  if ((rc = vcycle<VT, XT, ACC, SCALE>(c, l + 1, rc_v, xc_v))) return rc;
+  // the coarse level's x may hold anything: a V-cycle on a non-finite input
+  // leaves non-finite values in the context's coarse vectors
+  if ((rc = vcycle<VT, XT, ACC, SCALE>(c, l + 1, rc_v, xc_v, nullptr, nullptr, true))) return rc;
   k_prolong<XT, ACC, SCALE><<<tiles(nc), kBlock, 0, c->stream>>>(x, xc_v, L.inj, L.sigma, r0, nc, L.frow);
   HPCGX_CHECK_LAUNCH(c);
   return symgs_launch<VT, XT, ACC>(c, l, r, x, false, rz, nrz);
@@ -235,8 +239,20 @@ int mg_fp64(Ctx *c, int level, const double *r, double *x, double *rz, int64_t *
   return vcycle<double, double, double, false>(c, level, r, x, rz, nrz, untrusted_x);
 }
 
-int mg_fp32(Ctx *c, int level, const float *r, float *x) {
-  return vcycle<float, float, float, false>(c, level, r, x);
+int mg_fp32(Ctx *c, int level, const float *r, float *x, bool untrusted_x) {
+  return vcycle<float, float, float, false>(c, level, r, x, nullptr, nullptr, untrusted_x);
+}
+
+int zero_vec(Ctx *c, double *x, int64_t n) {
+  k_zero<double><<<tiles(n), kBlock, 0, c->stream>>>(x, n);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+int zero_vec(Ctx *c, float *x, int64_t n) {
+  k_zero<float><<<tiles(n), kBlock, 0, c->stream>>>(x, n);
+  HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale(const float *__restrict__ a, const float *__restrict__ s,
@@ -253,12 +269,12 @@ int mg_fp16_scaled(Ctx *c, const float *rhat, float *xhat) {
 
 // Scaled fp16 V-cycle on level 0: rhat = dinv r, V-cycle with D^-1/2 A D^-1/2,
 // x = dinv xhat.
-int mg_fp16(Ctx *c, const float *r, float *x) {
+int mg_fp16(Ctx *c, const float *r, float *x, bool untrusted_x) {
   Level &L = c->levels[0];
   const int64_t np = L.lay.npad;
   k_scale<<<tiles(np), kBlock, 0, c->stream>>>(r, L.dinv, L.r16, np);
   HPCGX_CHECK_LAUNCH(c);
-  int rc = vcycle<__half, float, float, true>(c, 0, L.r16, x);
+  int rc = vcycle<__half, float, float, true>(c, 0, L.r16, x, nullptr, nullptr, untrusted_x);
   if (rc) return rc;
   k_scale<<<tiles(np), kBlock, 0, c->stream>>>(x, L.dinv, x, np);
   HPCGX_CHECK_LAUNCH(c);
@@ -381,7 +397,8 @@ int hpcgx_mg_mixed(hpcgx_ctx *c, int precision, const double *r, double *x) {
   const int64_t np = c->levels[0].lay.npad;
   k_to_single<<<tiles(np), kBlock, 0, c->stream>>>(r, c->cg_r32, np);
   HPCGX_CHECK_LAUNCH(c);
-  rc = precision == HPCGX_PREC_FP32 ? mg_fp32(c, 0, c->cg_r32, c->cg_z32) : mg_fp16(c, c->cg_r32, c->cg_z32);
+  rc = precision == HPCGX_PREC_FP32 ? mg_fp32(c, 0, c->cg_r32, c->cg_z32, true)
+                                    : mg_fp16(c, c->cg_r32, c->cg_z32, true);
   if (rc) return rc;
   k_to_double<<<tiles(np), kBlock, 0, c->stream>>>(c->cg_z32, x, np);
   HPCGX_CHECK_LAUNCH(c);

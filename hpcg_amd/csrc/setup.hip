@@ -135,7 +135,7 @@ int ensure_reduce(Ctx *c, int64_t grid) {
 }
 
 // One-block finishing kernel of a reduction over `grid` producer blocks.
-int finish(Ctx *c, int nv, int64_t grid, int op) {
+int finish(Ctx *c, int nv, int64_t grid, int op, double *slot) {
   const double *src = c->partials;
   if (grid > kFoldAbove) {
     const int64_t g2 = (grid + kFold - 1) / kFold;
@@ -147,9 +147,9 @@ int finish(Ctx *c, int nv, int64_t grid, int op) {
     grid = g2;
   }
   if (nv == 1)
-    k_finish<1><<<1, kFinishThreads, 0, c->stream>>>(src, grid, c->scal, op);
+    k_finish<1><<<1, kFinishThreads, 0, c->stream>>>(src, grid, c->scal, op, slot);
   else
-    k_finish<2><<<1, kFinishThreads, 0, c->stream>>>(src, grid, c->scal, op);
+    k_finish<2><<<1, kFinishThreads, 0, c->stream>>>(src, grid, c->scal, op, slot);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -1116,6 +1116,7 @@ int hpcgx_destroy(hpcgx_ctx *c) {
   (void)hipFree(c->up_err);
   for (auto &ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : c->evpool) (void)hipEventDestroy(ev);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return HPCGX_OK;

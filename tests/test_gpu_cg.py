@@ -40,6 +40,80 @@ def test_cg_history_matches_oracle(hip, n):
     assert np.max(np.abs(x - 1.0)) < 1e-8   # converged to the exact solution (all ones)
 
 
+@pytest.mark.parametrize("dims,levels,precond", [
+    ((1, 1, 1), 1, True),     # zero norm at iteration 1
+    ((2, 2, 2), 1, False),    # b is an eigenvector: zero norm at iteration 1
+    ((8, 8, 8), 2, True),     # stop in the middle of the second batch
+    ((8, 8, 8), 1, False),    # stop in the fourth batch
+    ((16, 16, 16), 4, True),  # NaN norm (0/0 once r.z underflows)
+])
+def test_tolerance_zero_stops_like_the_reference(hip, dims, levels, precond):
+    """Tolerance 0 (the benchmark's timed sets) forces max_iter iterations
+    unless a norm is zero or NaN.  That test runs on the device and the host
+    reads the norms once per batch of 64 iterations; the solve must still end
+    at the first such norm with x as that iteration left it (an update after
+    it would turn x into NaN).  Tolerance 5e-324 takes the path that waits
+    for every norm and stops on the same iterations (normr0 <= 1.4e154, so a
+    nonzero normr/normr0 is far above 5e-324): the two agree bit for bit,
+    also when the first solve ended on a NaN and the batch's remaining
+    iterations ran the V-cycle on a NaN residual (the next solve must not
+    see it).  Whether a long solve ends on a zero or on a NaN (0/0 once r.z
+    underflows) depends on the dots' summation order, and so does the
+    iteration, by one or two."""
+    ctx = hip.Context(0)
+    ctx.generate(*dims, levels=levels)
+    P = po.Problem(*dims, levels=levels)
+    b = P.rhs()[0]
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    got = ctx.cg(db, dx, max_iter=300, tol=0.0, precond=precond, history=True)
+    x = dx.download()
+    ctx.fill(dx, 0.0)
+    eager = ctx.cg(db, dx, max_iter=300, tol=5e-324, precond=precond, history=True)
+    assert got["niters"] == eager["niters"] < 300
+    assert np.array_equal(got["hist"], eager["hist"], equal_nan=True)
+    assert np.array_equal(x, dx.download(), equal_nan=True)
+    want = P.cg(b, max_iter=300, tol=0.0, precond=precond, mode=2, history=True)
+    if want["niters"] == 1:   # one iteration: no dot-order difference yet
+        assert got["niters"] == 1 and got["normr"] == 0.0 and np.array_equal(x, want["x"])
+    else:
+        assert abs(got["niters"] - want["niters"]) <= 5
+    if got["normr"] == 0.0:
+        assert np.max(np.abs(x - 1.0)) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+def test_vcycle_after_a_nan_input(hip, prec):
+    """A V-cycle on a residual holding a NaN leaves NaN in the context's
+    coarse-level vectors (and in the mixed V-cycles' level-0 buffer); the next
+    V-cycle on a finite residual must give what a fresh context gives, as the
+    reference's, which zeroes x first (ComputeMG_ref.cpp:42), would."""
+    n = 16
+    r = rng(11).random(n ** 3) - 0.5
+    bad = r.copy()
+    bad[100] = np.nan
+    outs = []
+    for poison in (False, True):
+        ctx = device_problem(hip, n)
+        dz = ctx.vec(0)
+        if prec == "fp64":
+            if poison:
+                ctx.mg(0, ctx.vec(0, bad), dz)
+                assert np.isnan(dz.download()).any()
+            ctx.mg(0, ctx.vec(0, r), dz)
+        else:
+            code = hip.HPCGX_PREC_FP32 if prec == "fp32" else hip.HPCGX_PREC_FP16
+            ctx.prepare(code)
+            if poison:
+                ctx.mg_mixed(code, ctx.vec(0, bad), dz)
+                assert np.isnan(dz.download()).any()
+            ctx.mg_mixed(code, ctx.vec(0, r), dz)
+        outs.append(dz.download())
+        ctx.close()
+    assert np.isfinite(outs[1]).all()
+    assert np.array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("n,ref_reduction", [(16, 3.66234e-41), (32, 1.0298266781532878e-20)])
 def test_cg_reaches_reference_reduction(hip, n, ref_reduction):
     """The reference (natural-order Gauss-Seidel) reaches `ref_reduction`
