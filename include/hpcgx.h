@@ -198,7 +198,9 @@ int hpcgx_waxpby_dot(hpcgx_ctx *ctx, int64_t n, double alpha, const double *x, d
  * residual ||b - A x|| / normr0 <= tol, and need hpcgx_prepare_precision.
  * times: NULL or the reference's times[] (CG.cpp:133-141 slots 0..5,
  * accumulated).  hist: NULL or max_iter+1 doubles, normr after each
- * iteration (hist[0] = normr0). */
+ * iteration (hist[0] = normr0).  With tol = 0 the FP64 variant stops only
+ * on a norm that is zero or NaN; it runs that test on the device and waits
+ * for the host once per 64 iterations (results are the same). */
 int hpcgx_cg(hpcgx_ctx *ctx, int variant, const double *b, double *x, int max_iter, double tol,
              int do_preconditioning, int *niters, double *normr, double *normr0, double *times,
              double *hist);
