@@ -82,6 +82,24 @@ def test_tolerance_zero_stops_like_the_reference(hip, dims, levels, precond):
     ctx.close()
 
 
+@pytest.mark.parametrize("tol", [0.0, 1e-300])
+def test_cg_phase_times(hip, tol):
+    """times[] as CG.cpp:133-141 fills it: [0] the whole call, [1] dot,
+    [2] WAXPBY, [3] SpMV, [5] MG, accumulated over the iterations; with
+    tolerance 0 the phase events of each batch of iterations are read after
+    the batch (100 iterations: two batches)."""
+    n = 32
+    ctx = device_problem(hip, n)
+    b = po.Problem(n, n, n).rhs()[0]
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    t = np.zeros(10)
+    out = ctx.cg(db, dx, max_iter=100, tol=tol, times=t)
+    assert out["niters"] == 100
+    assert all(t[i] > 0 for i in (0, 1, 2, 3, 5)), t
+    assert t[1] + t[2] + t[3] + t[5] <= t[0] * 1.0001
+    assert t[5] > t[3]   # the V-cycle costs several SpMVs
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
 def test_vcycle_after_a_nan_input(hip, prec):
     """A V-cycle on a residual holding a NaN leaves NaN in the context's
