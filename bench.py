@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 REF_REDUCTION = {16: 3.66234e-41, 192: 5.31511e-05}
 # HBM bytes per launch of the fine-level colour step, from the calibrated
 # FETCH_SIZE / WRITE_SIZE runs in profiles/k_gs_fine_192_fp64_pmc_traffic.md
-TRAFFIC_PER_LAUNCH = {192: 359538215}
+TRAFFIC_PER_LAUNCH = {192: 361617890}
 
 
 def log(*a):
