@@ -24,14 +24,16 @@ __device__ __forceinline__ size_t ell_base(int64_t row) {
 
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD works on
 // one contiguous range of tiles and neighbouring rows share that XCD's L2.
-// ROT (the fine level's Gauss-Seidel steps): XCD x starts its range at x/8
-// of it and wraps.  A step's 8 ranges are slabs of one colour block, equally
-// spaced in memory, and swept in lockstep they meet the same HBM channels at
-// the same time; rotated, the fine 192^3 step takes 57.6 us instead of 59.3
-// (4 alternating processes each).  Whole-level passes (SpMV) keep the plain
-// map: there XCD x sweeps block x, and in lockstep the 8 XCDs gather x around
-// the same grid position, which the last-level cache serves once (rotated,
-// the 512^3 SpMV took 9.2-9.6 ms instead of 8.1-8.5).
+// ROT (the fine level's fp64 Gauss-Seidel steps): XCD x starts its range at
+// x/8 of it and wraps.  A step's 8 ranges are slabs of one colour block,
+// equally spaced in memory, and swept in lockstep they meet the same HBM
+// channels at the same time; rotated, the fine 192^3 step takes 57.6 us
+// instead of 59.3 (4 alternating processes each).  Whole-level passes (SpMV)
+// keep the plain map: there XCD x sweeps block x, and in lockstep the 8 XCDs
+// gather x around the same grid position, which the last-level cache serves
+// once (rotated, the 512^3 SpMV took 9.2-9.6 ms instead of 8.1-8.5).  The
+// fp32 / fp16 steps keep it too (rotated, the mixed 512^3 solves ran 1.7 %
+// slower).
 template <bool ROT = false>
 __device__ __forceinline__ int64_t tile_id() {
   const int64_t G = gridDim.x, b = blockIdx.x;
@@ -388,7 +390,7 @@ void k_gs(const ColIdx ci, const VT *__restrict__ vals,
   static_assert(!RZ || ZK < 0, "r.z is taken from the post-smoother");
   // blocks of 256 threads, or of 64 for the small coarse fp64 steps (one wave
   // per block spreads them over more CUs); the r.z form always takes 256
-  const int64_t t = tile_id<FINE>() * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = tile_id<FINE && sizeof(VT) == 8>() * (int64_t)blockDim.x + threadIdx.x;
   if constexpr (RZ) {
     // every thread reaches the block reduction; rows past the block add 0
     double acc = 0.0;
