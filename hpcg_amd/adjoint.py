@@ -94,7 +94,12 @@ class KappaOperator:
 
 
 class KappaSolve(torch.autograd.Function):
-    """x = A(kappa)^-1 b with gradients for kappa and b."""
+    """x = A(kappa)^-1 b with gradients for kappa and b.
+
+    kappa may be a tensor of any shape with nx*ny*nz entries (natural order),
+    on any device, or a numpy array (then it gets no gradient); each gradient
+    is returned in its input's shape, dtype and device, and only when autograd
+    asks for it."""
 
     @staticmethod
     def forward(fctx, kappa, b, grid, tol=1e-10, variant="mixed_fp32"):
@@ -102,6 +107,7 @@ class KappaSolve(torch.autograd.Function):
         op = KappaOperator(kappa, grid, variant=variant, device=b.device.index or 0)
         x = op.solve(b, tol=tol)
         fctx.op, fctx.tol = op, tol
+        fctx.kappa_meta = (kappa.shape, kappa.dtype, kappa.device) if isinstance(kappa, torch.Tensor) else None
         fctx.save_for_backward(x)
         fctx.timing = {"forward_s": time.perf_counter() - t, "setup_s": op.setup_s,
                        "forward_solve_s": op.last["seconds"], "forward_iters": op.last["niters"]}
@@ -113,14 +119,25 @@ class KappaSolve(torch.autograd.Function):
         (x,) = fctx.saved_tensors
         t = time.perf_counter()
         op = fctx.op
-        lam = op.solve(gx.reshape(-1), tol=fctx.tol)
-        tg = time.perf_counter()
-        gk = op.grad_kappa(lam, x)
-        fctx.timing.update({"backward_s": time.perf_counter() - t, "adjoint_solve_s": op.last["seconds"],
-                            "adjoint_iters": op.last["niters"], "grad_kernel_s": time.perf_counter() - tg})
-        KappaSolve.last_timing = fctx.timing
-        op.close()
-        return gk, lam.reshape(gx.shape), None, None, None
+        need_k = fctx.needs_input_grad[0] and fctx.kappa_meta is not None
+        need_b = fctx.needs_input_grad[1]
+        gk = gb = None
+        try:
+            if need_k or need_b:
+                lam = op.solve(gx.reshape(-1), tol=fctx.tol)
+                fctx.timing.update({"adjoint_solve_s": op.last["seconds"], "adjoint_iters": op.last["niters"]})
+                if need_b:
+                    gb = lam.reshape(gx.shape).to(gx.dtype)
+                if need_k:
+                    tg = time.perf_counter()
+                    shape, dtype, device = fctx.kappa_meta
+                    gk = op.grad_kappa(lam, x).reshape(shape).to(device=device, dtype=dtype)
+                    fctx.timing["grad_kernel_s"] = time.perf_counter() - tg
+            fctx.timing["backward_s"] = time.perf_counter() - t
+            KappaSolve.last_timing = fctx.timing
+        finally:
+            op.close()
+        return gk, gb, None, None, None
 
 
 KappaSolve.last_timing = {}

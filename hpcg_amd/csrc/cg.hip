@@ -317,22 +317,22 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
         int64_t nrz = 0;
         if ((rc = mg_fp64(c, 0, r, z, c->partials, &nrz))) return rc;
         tm.mark(1);
-        finish(c, 1, nrz, j == 1 ? F_RZ_FIRST : F_RZ);
+        if ((rc = finish(c, 1, nrz, j == 1 ? F_RZ_FIRST : F_RZ))) return rc;
       } else {
         if (precond && (rc = mg_fp64(c, 0, r, z))) return rc;
         tm.mark(1);
         k_cg_rz<double><<<G, kBlock, 0, c->stream>>>(np, r, z, c->partials);
-        finish(c, 1, G, j == 1 ? F_RZ_FIRST : F_RZ);
+        if ((rc = finish(c, 1, G, j == 1 ? F_RZ_FIRST : F_RZ))) return rc;
       }
       tm.mark(2);
       k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, j == 1);
       tm.mark(3);
       k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p,
                                                          Ap, np, c->partials);
-      finish(c, 1, T, F_PAP);
+      if ((rc = finish(c, 1, T, F_PAP))) return rc;
       tm.mark(4);
       k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->scal);
-      finish(c, 1, G, F_NORMR2, deferred ? c->hist + (j - k) : nullptr);
+      if ((rc = finish(c, 1, G, F_NORMR2, deferred ? c->hist + (j - k) : nullptr))) return rc;
       tm.mark(5);
       HPCGX_CHECK_LAUNCH(c);
     }
@@ -408,7 +408,7 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
     if (rc) return rc;
     tm.mark(1);
     k_fcg_rz<<<G, kBlock, 0, c->stream>>>(np, r, z32, Ap, c->partials, restart ? 1 : 0, dscale);
-    finish(c, 2, G, restart ? F_RZ_FIRST : F_FCG);
+    if ((rc = finish(c, 2, G, restart ? F_RZ_FIRST : F_FCG))) return rc;
     tm.mark(2);
     if (restart)
       k_copy_z<<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, dscale);
@@ -422,10 +422,10 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
     else
       k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p, Ap,
                                                         np, c->partials);
-    finish(c, 1, T, F_PAP);
+    if ((rc = finish(c, 1, T, F_PAP))) return rc;
     tm.mark(4);
     k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal, dscale);
-    finish(c, 1, G, F_NORMR2);
+    if ((rc = finish(c, 1, G, F_NORMR2))) return rc;
     tm.mark(5);
     HPCGX_CHECK_LAUNCH(c);
     if ((rc = read_scalar(c, S_NORMR2, &n2))) return rc;

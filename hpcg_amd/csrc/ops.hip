@@ -405,18 +405,23 @@ int hpcgx_mg_mixed(hpcgx_ctx *c, int precision, const double *r, double *x) {
   return HPCGX_OK;
 }
 
+// The vector hooks take the reference's n = nrow (ComputeDotProduct.cpp:41,
+// ComputeWAXPBY.cpp:42: the whole level-0 vector).  Device vectors are in
+// colour order, where the first k entries are not the first k natural rows,
+// so any other length has no reference meaning and is refused; npad (the
+// padded length, padding entries zero) is accepted as the same vector.
 static int check_n(hpcgx_ctx *c, int64_t n, const char *who) {
   int rc = check_level(c, 0, who);
   if (rc) return rc;
-  if (n < 0 || n > c->levels[0].lay.npad) return set_err(c, HPCGX_EINVAL, std::string(who) + ": bad length");
+  const Layout &lay = c->levels[0].lay;
+  if (n != lay.n && n != lay.npad)
+    return set_err(c, HPCGX_EINVAL, std::string(who) + ": n must be the level-0 row count (" +
+                                        std::to_string((long long)lay.n) + ")");
   return HPCGX_OK;
 }
 
-// Length n refers to the level-0 device vector; n equal to the level's true
-// row count covers the whole padded vector (padding entries are zero).
-static int64_t span(hpcgx_ctx *c, int64_t n) {
-  return n == c->levels[0].lay.n ? c->levels[0].lay.npad : n;
-}
+// n = the level's row count covers the whole padded vector.
+static int64_t span(hpcgx_ctx *c, int64_t) { return c->levels[0].lay.npad; }
 
 int hpcgx_dot(hpcgx_ctx *c, int64_t n, const double *x, const double *y, double *result) {
   int rc = check_n(c, n, "dot");

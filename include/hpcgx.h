@@ -176,11 +176,14 @@ int hpcgx_mg(hpcgx_ctx *ctx, int level, const double *r, double *x);
  * (HPCGX_PREC_FP16) copies of every level, fp32 arithmetic, x widened to fp64.
  * Level-0 device vectors; needs hpcgx_prepare_precision. */
 int hpcgx_mg_mixed(hpcgx_ctx *ctx, int precision, const double *r, double *x);
-/* result = x . y over n entries of a level-0 vector (ComputeDotProduct,
+/* result = x . y over a whole level-0 vector, n = its row count (ComputeDotProduct,
  * ComputeDotProduct.cpp:41-47).  Fixed-order reduction: the same inputs give
  * the same bits on every run. */
 int hpcgx_dot(hpcgx_ctx *ctx, int64_t n, const double *x, const double *y, double *result);
-/* w = alpha x + beta y (ComputeWAXPBY, ComputeWAXPBY.cpp:42-48; bit-exact). */
+/* w = alpha x + beta y over a whole level-0 vector, n = its row count
+ * (ComputeWAXPBY, ComputeWAXPBY.cpp:42-48; bit-exact).  The vector calls
+ * refuse any other n (HPCGX_EINVAL): device vectors are in colour order, so a
+ * shorter length would not select the reference's first n rows. */
 int hpcgx_waxpby(hpcgx_ctx *ctx, int64_t n, double alpha, const double *x, double beta,
                  const double *y, double *w);
 /* w = alpha x + beta y and result = w . w in one pass (no reference counterpart;

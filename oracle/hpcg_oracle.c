@@ -693,6 +693,9 @@ static void ensure_f32(orc_level *L) {
   }
 }
 
+static int planted_fault = 0;
+void orc_plant_fault(int which) { planted_fault = which; }
+
 /* Symmetric diagonal scaling: ahat_ij = a_ij * dinv_i * dinv_j (in double,
  * then rounded to fp16), dinv = 1/sqrt(a_ii); the diagonal becomes 1. */
 static void ensure_f16(orc_level *L) {
@@ -722,7 +725,8 @@ static void ensure_f16(orc_level *L) {
       const int f = l->f2c[i];
       const double df = l->vals[(int64_t)f * NSLOT + l->diag_slot[f]];
       const double dc = l->coarse->vals[(int64_t)i * NSLOT + l->coarse->diag_slot[i]];
-      l->sigma[i] = (float)sqrt(df / dc);
+      const double sg = planted_fault == 1 ? sqrt(dc / df) : planted_fault == 2 ? 1.0 : sqrt(df / dc);
+      l->sigma[i] = (float)sg;
     }
   }
 }

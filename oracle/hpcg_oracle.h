@@ -28,8 +28,16 @@
  *                                           exact dense solve (numpy) on tiny
  *                                           grids
  *   mixed-precision MG / flexible CG     -> fp32 MG == fp64 MG to fp32
- *                                           rounding; converged solutions
- *                                           checked by an fp64 true residual
+ *                                           rounding; scaled fp16 MG == fp64
+ *                                           MG to fp16 rounding on
+ *                                           hierarchies whose transfer factor
+ *                                           sigma = sqrt(d_f/d_c) is far from
+ *                                           1 (each level scaled by its own
+ *                                           random diagonal) and on the kappa
+ *                                           operator, a check that planted
+ *                                           sigma mistakes are shown to fail;
+ *                                           converged solutions checked by an
+ *                                           fp64 true residual
  *   general levels (orc_create_rows,     -> natural-order spmv / symgs / mg /
  *   orc_link)                               cg bit-exact against the
  *                                           reference build with the same
@@ -122,6 +130,11 @@ int orc_cg_mixed(orc_level *L, const double *b, double *x, int max_iter, double 
 /* Variable-coefficient adjoint: g_k = -1/2 sum_{j in N(k)} (lam_k-lam_j)(x_k-x_j)
  *                                     - (26-n_k) lam_k x_k                      */
 void orc_kappa_grad(const orc_level *L, const double *lam, const double *x, double *g);
+
+/* Plants a known mistake in the scaled fp16 V-cycle so that a test can show
+ * that its check catches it: 0 none, 1 sigma inverted (sqrt(d_c/d_f)),
+ * 2 sigma dropped (1).  Applies to fp16 copies built afterwards.  Tests only. */
+void orc_plant_fault(int which);
 
 /* fp16 helpers (round to nearest even) shared with tests */
 uint16_t orc_f32_to_f16(float f);

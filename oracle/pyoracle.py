@@ -66,6 +66,7 @@ def lib():
                              C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
         L.orc_mg_f32.argtypes = [vp, _fp, _fp]
         L.orc_mg_f16.argtypes = [vp, _fp, _fp]
+        L.orc_plant_fault.argtypes = [C.c_int]
         L.orc_cg_mixed.restype = C.c_int
         L.orc_cg_mixed.argtypes = [vp, _dp, _dp, C.c_int, C.c_double, C.c_int,
                                    C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
@@ -283,6 +284,24 @@ class Problem(Level):
         if history:
             out["hist"] = hist[: it.value + 1]
         return out
+
+
+class planted_fault:
+    """Context manager: the oracle's scaled fp16 V-cycle carries a known
+    mistake (orc_plant_fault) for hierarchies built inside the block, so a
+    test can show that its check catches it.  1: sigma inverted; 2: sigma
+    dropped."""
+
+    def __init__(self, which):
+        self.which = int(which)
+
+    def __enter__(self):
+        lib().orc_plant_fault(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant_fault(0)
+        return False
 
 
 def dot(x, y):

@@ -1,0 +1,131 @@
+"""The FP64 drop-in workload at its real size: the 192^3 HPCG problem
+(7,077,888 rows, 189,119,224 nonzeros on the fine level, 4 levels), where
+the reference's own numbers are known (its 50-iteration residual reduction
+5.31511e-05 and TestCG's unpreconditioned count at the bound of 12, measured
+from the reference build).
+
+* the generated matrix of every level, SpMV, the coloured symmetric
+  Gauss-Seidel step and the V-cycle, bit for bit against the oracle;
+* the 50-iteration CG residual history against the oracle's (same
+  recurrence, CG.cpp:86-130; only the dots' summation order differs);
+* the reference's 50-iteration reduction reached within 50 +/- 2 iterations
+  (main.cpp:281-293 accepts up to 10x);
+* TestCG (TestCG.cpp:84-102: <= 12 unpreconditioned, <= 2 preconditioned
+  iterations), TestSymmetry (TestSymmetry.cpp:76-132) and TestNorms
+  (TestNorms.cpp:31-44) restated over the device.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+G = 192
+REF_REDUCTION_192 = 5.31511e-05   # CG_ref, 50 iterations, natural-order Gauss-Seidel
+
+
+@pytest.fixture(scope="module")
+def p192(hip):
+    ctx = hip.Context(0)
+    ctx.generate(G, G, G, levels=4)
+    P = po.Problem(G, G, G)
+    yield ctx, P
+    ctx.close()
+
+
+def test_192_matrix_levels_bitexact(p192):
+    ctx, P = p192
+    for l in range(4):
+        onr, ocols, ovals, _ = P.level(l).arrays()
+        nr, cols, vals = ctx.export_level(l)
+        mask = np.arange(27)[None, :] < onr[:, None]
+        assert np.array_equal(nr, onr), l
+        assert np.array_equal(cols[mask], ocols[mask]), l
+        assert np.array_equal(vals[mask], ovals[mask]), l
+
+
+def test_192_spmv_symgs_vcycle_bitexact(p192):
+    ctx, P = p192
+    g = np.random.default_rng(192)
+    x = g.random(P.n) - 0.5
+    r = g.random(P.n)
+    dx, dr, dy = ctx.vec(0, x), ctx.vec(0, r), ctx.vec(0)
+    ctx.spmv(0, dx, dy)
+    assert np.array_equal(dy.download(), P.spmv(x))
+    ctx.symgs(0, dr, dx)
+    assert np.array_equal(dx.download(), P.symgs(r, x, mode=2))
+    ctx.mg(0, dr, dy)
+    assert np.array_equal(dy.download(), P.mg(r, mode=2))
+
+
+def test_192_cg_history_and_reference_reduction(p192):
+    ctx, P = p192
+    b = P.rhs()[0]
+    db, dx = ctx.vec(0, b), ctx.vec(0)
+    got = ctx.cg(db, dx, max_iter=50, tol=0.0, history=True)
+    want = P.cg(b, max_iter=50, mode=2, history=True)
+    assert got["niters"] == want["niters"] == 50
+    h, hw = got["hist"] / got["normr0"], want["hist"] / want["normr0"]
+    assert np.allclose(h, hw, rtol=1e-8, atol=0), np.max(np.abs(h / hw - 1))
+    ctx.fill(dx, 0.0)
+    out = ctx.cg(db, dx, max_iter=52, tol=REF_REDUCTION_192)
+    assert out["normr"] / out["normr0"] <= REF_REDUCTION_192
+    assert out["niters"] <= 52, out["niters"]
+
+
+def test_192_testcg(p192):
+    """TestCG.cpp:62-115: diagonal and b exaggerated (x 1e6, the first 9
+    rows further), two unpreconditioned and two preconditioned solves at
+    1e-12; the reference's own run sits at the bound, 12 / 1."""
+    ctx, P = p192
+    n = P.n
+    b = P.rhs()[0]
+    orig = ctx.get_diagonal(0)
+    scale = np.full(n, 1.0e6)
+    gid = np.arange(9)
+    scale[gid] = (gid + 2) * 1.0e6
+    ctx.set_diagonal(0, orig * scale)
+    try:
+        db, dx = ctx.vec(0, b * scale), ctx.vec(0)
+        it = {}
+        for prec in (False, True):
+            it[prec] = []
+            for _ in range(2):
+                ctx.fill(dx, 0.0)
+                it[prec].append(ctx.cg(db, dx, max_iter=50, tol=1e-12, precond=prec)["niters"])
+    finally:
+        ctx.set_diagonal(0, orig)
+    assert np.array_equal(ctx.get_diagonal(0), orig)
+    assert max(it[False]) <= 12 and max(it[True]) <= 2, it
+
+
+def test_192_testsymmetry_and_testnorms(p192):
+    ctx, P = p192
+    eps = np.finfo(float).eps
+    g = np.random.default_rng(7)
+    x, y = g.random(P.n) + 1.0, g.random(P.n) + 1.0
+    dx, dy, dz = ctx.vec(0, x), ctx.vec(0, y), ctx.vec(0)
+    xn2, yn2 = ctx.dot(dx, dx), ctx.dot(dy, dy)
+    scale = (xn2 * 52.0 * yn2 + yn2 * 52.0 * xn2) * eps   # TestSymmetry.cpp:80,98,118
+    ctx.spmv(0, dy, dz)
+    xAy = ctx.dot(dx, dz)
+    ctx.spmv(0, dx, dz)
+    yAx = ctx.dot(dy, dz)
+    assert abs(xAy - yAx) / scale <= 1.0
+    ctx.mg(0, dy, dz)
+    xMy = ctx.dot(dx, dz)
+    ctx.mg(0, dx, dz)
+    yMx = ctx.dot(dy, dz)
+    assert abs(xMy - yMx) / scale <= 1.0
+    b, _, xe = P.rhs()
+    dx.upload(xe)
+    ctx.spmv(0, dx, dz)
+    assert np.array_equal(dz.download(), b)   # A xexact == b exactly (TestSymmetry.cpp:122-132)
+    db = ctx.vec(0, b)
+    vals = []
+    for _ in range(3):
+        ctx.fill(dx, 0.0)
+        out = ctx.cg(db, dx, max_iter=50, tol=0.0)
+        vals.append(out["normr"] / out["normr0"])
+    assert len(set(vals)) == 1   # TestNorms: variance 0

@@ -14,16 +14,17 @@ BINARY = os.path.join(ROOT, "oracle", "_ref", "xhpcg_mi355x")
 pytestmark = pytest.mark.gpu
 
 
-def run_driver(tmp_path, n):
+def run_driver(tmp_path, n, threads=1):
     if not os.path.exists(BINARY):
         pytest.skip("reference driver not built (needs the reference sources: make -C oracle ref)")
-    # One thread: the reference's own CG_ref (which sets the residual reduction
-    # the device CG must reach) then sums its dots in a fixed order.  Its
-    # OpenMP reduction does not, and at 16^3 the target (3.66e-41) is in the
-    # rounding noise, so with threads it moves from run to run.
-    env = dict(os.environ, OMP_NUM_THREADS="1")
+    # One thread at small grids: the reference's own CG_ref (which sets the
+    # residual reduction the device CG must reach) then sums its dots in a
+    # fixed order.  Its OpenMP reduction does not, and at 16^3 the target
+    # (3.66e-41) is in the rounding noise, so with threads it moves from run
+    # to run.  At 192^3 the target (5.3e-5) is far above it.
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     p = subprocess.run([BINARY, str(n), str(n), str(n), "--rt=0"], cwd=tmp_path, env=env,
-                       capture_output=True, text=True, timeout=300)
+                       capture_output=True, text=True, timeout=900)
     assert p.returncode == 0, p.stderr[-2000:]
     reports = [f for f in os.listdir(tmp_path) if f.startswith("HPCG-Benchmark")]
     assert len(reports) == 1, (os.listdir(tmp_path), p.stderr[-2000:])
@@ -47,3 +48,23 @@ def test_reference_driver_valid_with_dropin_hooks(tmp_path, n):
     assert kv["Reproducibility Information::Result"] == "PASSED"
     assert float(kv["Reproducibility Information::Scaled residual variance"]) == 0.0
     assert "Reference version of" not in text   # every hook is the device one
+
+
+@pytest.mark.timeout(900)
+def test_reference_driver_valid_at_192(tmp_path):
+    """The drop-in workload's own grid.  The reference's CG_ref runs 50
+    iterations on the host first (its Gauss-Seidel is serial), so this takes
+    minutes; the device part takes seconds."""
+    text, kv = run_driver(tmp_path, 192, threads=16)
+    assert "HPCG result is VALID" in text, text[-3000:]
+    for key in ("Spectral Convergence Tests::Result", "Iteration Count Information::Result",
+                "Reproducibility Information::Result",
+                "Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon::Result"):
+        assert kv[key] == "PASSED", key
+    assert int(kv["Spectral Convergence Tests::Unpreconditioned::Maximum iteration count"]) <= 12
+    assert int(kv["Spectral Convergence Tests::Preconditioned::Maximum iteration count"]) <= 2
+    assert int(kv["Iteration Count Information::Optimized CG iterations per set"]) <= 52
+    assert "Reference version of" not in text
+    print("192^3 rating", kv["Final Summary::HPCG result is VALID with a GFLOP/s rating of"],
+          "raw", kv["GFLOP/s Summary::Raw Total"], "setup", kv["Setup Information::Setup Time"],
+          "optimisation", kv["Benchmark Time Summary::Optimization phase"])

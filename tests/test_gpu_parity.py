@@ -287,3 +287,23 @@ def test_wrong_size_vectors_are_refused(hip):
     with pytest.raises(hip.HpcgxError, match="vector holds"):
         ctx.cg(big, fresh, max_iter=1)
     ctx.close()
+
+
+def test_vector_hooks_take_the_row_count_only(hip):
+    """ComputeDotProduct / ComputeWAXPBY receive n = nrow (the whole vector).
+    Device vectors are in colour order, so a shorter n would select other
+    rows than the reference's first n: it is refused, before any launch."""
+    ctx = hip.Context(0)
+    ctx.generate(16, 16, 16)
+    n, npad = ctx.info(0)["n"], ctx.info(0)["npad"]
+    x, y, w = ctx.vec(0, np.ones(n)), ctx.vec(0, np.full(n, 2.0)), ctx.vec(0)
+    assert ctx.dot(x, y) == 2.0 * n
+    assert ctx.dot(x, y, n=npad) == 2.0 * n
+    for bad in (n - 1, 5, npad + 1, 0):
+        with pytest.raises(hip.HpcgxError, match="row count"):
+            ctx.dot(x, y, n=bad)
+        with pytest.raises(hip.HpcgxError, match="row count"):
+            ctx.waxpby(1.0, x, 1.0, y, w, n=bad)
+        with pytest.raises(hip.HpcgxError, match="row count"):
+            ctx.waxpby_dot(1.0, x, 1.0, y, w, n=bad)
+    ctx.close()

@@ -269,6 +269,69 @@ def test_mixed_mg_close_to_fp64():
     assert np.linalg.norm(z16 - z) <= 5e-3 * np.linalg.norm(z)
 
 
+def _rescaled_hpcg(n, seed, spread=1.0):
+    """The HPCG hierarchy with every level replaced by S_l A_l S_l, S_l an
+    independent random diagonal 10^U(-spread, spread): SPD, same pattern and
+    injection, but the fine and coarse diagonals at an injected point are no
+    longer equal, so sigma = sqrt(d_f/d_c) spans about 1e-2..1e2."""
+    P = po.Problem(n, n, n)
+    g = np.random.default_rng(seed)
+    out = []
+    for k in range(4):
+        nr, cols, vals, f2c = P.level(k).arrays()
+        s = 10.0 ** g.uniform(-spread, spread, len(nr))
+        live = np.arange(27)[None, :] < nr[:, None]
+        v = np.where(live, vals * s[:, None] * s[np.where(live, cols, 0)], 0.0)
+        out.append((nr, cols, v, None if f2c is None else f2c.copy()))
+    return out
+
+
+def _fp16_cases():
+    import general_matrices as gm
+    return [
+        # (name, builder, bar): the scaled fp16 V-cycle equals the fp64 one in
+        # exact arithmetic (Gauss-Seidel is invariant under D^-1/2 . D^-1/2,
+        # and the transfers carry sigma = sqrt(d_f/d_c)); what is left is fp16
+        # rounding of the scaled values.  Uniform entries (the rescaled HPCG
+        # matrix's -1/26) round alike, which leaves ~1.5e-3; random entries
+        # (kappa operator, random SPD rows) ~7e-5.
+        ("rescaled_hpcg_16", lambda: po.Problem.from_rows(_rescaled_hpcg(16, 1)), 5e-3),
+        ("kappa_16", lambda: po.Problem(16, 16, 16, kappa=np.exp(0.5 * np.random.default_rng(3).standard_normal(16 ** 3))), 5e-4),
+        ("kappa_24x8x16", lambda: po.Problem(24, 8, 16, kappa=np.exp(0.5 * np.random.default_rng(4).standard_normal(24 * 8 * 16))), 5e-4),
+        ("random_spd", lambda: po.Problem.from_rows(gm.random_spd(seed=0)), 5e-4),
+    ]
+
+
+def _fp16_error(P):
+    r = np.random.default_rng(6).random(P.n) - 0.5
+    z = P.mg(r)
+    z16 = P.mg_f16(r.astype(np.float32)).astype(np.float64)
+    return np.linalg.norm(z16 - z) / np.linalg.norm(z)
+
+
+@pytest.mark.parametrize("case", range(4), ids=["rescaled_hpcg_16", "kappa_16", "kappa_24x8x16", "random_spd"])
+def test_fp16_mg_matches_fp64_where_sigma_is_not_one(case):
+    """Pins the scaled fp16 V-cycle's transfer factor sigma = sqrt(d_f/d_c)
+    (orc_mg_f16) against the fp64 V-cycle (orc_mg, itself bit-exact against
+    the reference) on operators whose fine and coarse diagonals differ; on
+    the HPCG matrix sigma is 1 and would hide any mistake in it.  The bars
+    sit 3-12x above the fp16 rounding error measured on these cases; an
+    inverted or dropped sigma exceeds them (next test)."""
+    name, build, bar = _fp16_cases()[case]
+    P = build()
+    assert _fp16_error(P) <= bar, name
+
+
+@pytest.mark.parametrize("fault", [1, 2], ids=["sigma_inverted", "sigma_dropped"])
+def test_fp16_check_catches_planted_sigma_mistakes(fault):
+    """The check above fails on every case when the oracle's sigma is
+    sqrt(d_c/d_f) or 1 (mistakes planted in the oracle, on the CPU)."""
+    with po.planted_fault(fault):
+        for name, build, bar in _fp16_cases():
+            err = _fp16_error(build())
+            assert err > 2 * bar, (name, err)
+
+
 @pytest.mark.parametrize("prec", [32, 16])
 def test_mixed_cg_true_residual(prec):
     P = po.Problem(16, 16, 16)
