@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhpcgx.so
 
 HPCGX_PREC_FP64, HPCGX_PREC_FP32, HPCGX_PREC_FP16 = 0, 1, 2
 HPCGX_CG_FP64, HPCGX_CG_MIXED_FP32, HPCGX_CG_MIXED_FP16 = 0, 1, 2
+HPCGX_COLS_EXPLICIT, HPCGX_COLS_COMPACT = 0, 1
 
 ERRORS = {0: "ok", -1: "EINVAL", -2: "ENOTSTENCIL", -3: "EHIP", -4: "ENOMEM", -5: "ESTATE"}
 
@@ -41,6 +42,8 @@ SIGNATURES = {
     "hpcgx_level_get_diagonal": (C.c_int, [_vp, C.c_int, _dp]),
     "hpcgx_level_export_packed": (C.c_int, [_vp, C.c_int, _u8p, _ip, _dp]),
     "hpcgx_prepare_precision": (C.c_int, [_vp, C.c_int]),
+    "hpcgx_set_column_format": (C.c_int, [_vp, C.c_int]),
+    "hpcgx_level_column_forms": (C.c_int, [_vp, C.c_int, _i64p]),
     "hpcgx_memory_bytes": (C.c_double, [_vp]),
     "hpcgx_vec_alloc": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_void_p)]),
     "hpcgx_vec_free": (C.c_int, [_vp, _vp]),
@@ -229,6 +232,16 @@ class Context:
 
     def prepare(self, precision):
         self._call("hpcgx_prepare_precision", self.h, precision)
+
+    def set_column_format(self, fmt):
+        """HPCGX_COLS_COMPACT (default) or HPCGX_COLS_EXPLICIT."""
+        self._call("hpcgx_set_column_format", self.h, fmt)
+
+    def column_forms(self, level=0):
+        """Slices of a level per column form: {'delta', 'explicit', 'uniform'}."""
+        out = np.zeros(3, np.int64)
+        self._call("hpcgx_level_column_forms", self.h, level, out)
+        return {"delta": int(out[0]), "explicit": int(out[1]), "uniform": int(out[2])}
 
     def memory_bytes(self):
         return lib().hpcgx_memory_bytes(self.h)

@@ -56,8 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict
 }
 
 // Ap = A p and the partials of pAp = p.Ap.  One 256-row tile per block.
-// CI: column indices from the compact form (the mixed-precision solvers,
-// which already stand outside the reference's rules; values stay fp64).
+// CI: column indices from the compact form.
 template <bool CI>
 __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(double, CI) void k_cg_spmv_dot(const ColIdx ci, const double *__restrict__ vals,
                                                         const double *__restrict__ p,
@@ -68,20 +67,17 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(double, CI) void k_c
   if (row < nrows) {
     const size_t b = ell_base(row);
     const int slice = slice_of<true>(row);
-    auto row_sum = [&](auto cm) {
+    const double sum = with_cols<CI, true>(ci, slice, [&](auto cm) {
       constexpr int CM = decltype(cm)::value;
-      double sum = 0.0;
+      double acc_row = 0.0;
 #pragma unroll
       for (int s = 0; s < kSlots; ++s) {
         const int c = col_at<CM, true>(ci, row, slice, b, s);
         const double v = ld_val<double>(vals + b + s * kSlice);
-        sum += v * p[c];
+        acc_row += v * p[c];
       }
-      return sum;
-    };
-    double sum;
-    if constexpr (CI) sum = slice_wide<CI>(ci, slice) ? row_sum(cmode<0>{}) : row_sum(cmode<1>{});
-    else sum = row_sum(cmode<0>{});
+      return acc_row;
+    });
     Ap[row] = sum;
     acc = p[row] * sum;
   }
@@ -285,6 +281,7 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
   int rc;
   PhaseTimer tm{c, times != nullptr};
   const bool fuse_rz = precond && c->levels.size() > 1 && !L0.lay.general;
+  const ColIdx ci0 = col_idx(c, L0);
   // preamble: Ap = A x ; r = b - Ap ; normr = ||r||  (CG.cpp:86-90)
   if ((rc = spmv_fp64(c, 0, x, Ap))) return rc;
   k_cg_residual<false><<<G, kBlock, 0, c->stream>>>(np, b, Ap, r, nullptr, c->partials);
@@ -327,8 +324,10 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
       tm.mark(2);
       k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, j == 1);
       tm.mark(3);
-      k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p,
-                                                         Ap, np, c->partials);
+      if (ci0.cbase)
+        k_cg_spmv_dot<true><<<T, kBlock, 0, c->stream>>>(ci0, L0.vals, p, Ap, np, c->partials);
+      else
+        k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ci0, L0.vals, p, Ap, np, c->partials);
       if ((rc = finish(c, 1, T, F_PAP))) return rc;
       tm.mark(4);
       k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->scal);
@@ -416,12 +415,11 @@ static int cg_mixed(Ctx *c, int variant, const double *b, double *x, int max_ite
       k_cg_p<float><<<tiles(np), kBlock, 0, c->stream>>>(np, z32, p, c->scal, 0, dscale);
     restart = false;
     tm.mark(3);
-    if (L0.cbase)
-      k_cg_spmv_dot<true><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, L0.cbase, L0.cdelta, L0.cwide}, L0.vals, p,
-                                                       Ap, np, c->partials);
+    const ColIdx ci0 = col_idx(c, L0);
+    if (ci0.cbase)
+      k_cg_spmv_dot<true><<<T, kBlock, 0, c->stream>>>(ci0, L0.vals, p, Ap, np, c->partials);
     else
-      k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ColIdx{L0.cols, nullptr, nullptr, nullptr}, L0.vals, p, Ap,
-                                                        np, c->partials);
+      k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ci0, L0.vals, p, Ap, np, c->partials);
     if ((rc = finish(c, 1, T, F_PAP))) return rc;
     tm.mark(4);
     k_cg_update<true><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, r32, c->partials, c->scal, dscale);

@@ -22,6 +22,19 @@ constexpr int kColours = 8;     // parity classes of the stencil layout
 constexpr int kMaxColours = 64; // colours of a general (unstructured) level
 constexpr int kMaxLevels = 8;
 
+// Column forms of a 64-row slice in the compact index copy.  A zero-valued
+// slot (a neighbour outside the grid, an unused entry) may name any row of
+// the level, since its product is an exact zero; the diagonal always counts
+// as a stored entry.
+enum ColForm : uint8_t {
+  kFormDelta = 0,    // column = base[slot] + lane + int8 delta[slot][lane]
+  kFormWide = 1,     // some delta does not fit in int8: the explicit 32-bit column
+  kFormUniform = 2,  // every stored entry of each slot sits at base[slot] + lane: no per-entry index
+};
+
+// Column index format of a context (hpcgx_set_column_format).
+enum ColFormat : int { kColsExplicit = 0, kColsCompact = 1 };
+
 // Device scalars of the CG recurrences (Ctx::scal).
 enum Scalar : int {
   S_RTZ = 0,
@@ -136,12 +149,14 @@ struct Level {
   double *vals = nullptr;
   float *vals32 = nullptr;
   __half *vals16 = nullptr;
-  // compact column indices of the reduced-precision copies: column of
-  // (row, slot) = cbase[slice*27+slot] + lane + cdelta[ell], or cols[] when
-  // cwide[slice] is set (deltas out of int8 range)
+  // compact column indices (the default column format of every copy): per
+  // slice a form (cform[slice], ColForm) and per slice and slot a 32-bit
+  // base; the column of (row, slot) is cbase[slice*27+slot] + lane (uniform
+  // slices), that plus the int8 cdelta[ell] (delta slices), or cols[ell]
+  // (wide slices, whose deltas do not fit in int8)
   int *cbase = nullptr;
   int8_t *cdelta = nullptr;
-  uint8_t *cwide = nullptr;
+  uint8_t *cform = nullptr;
   float *dinv = nullptr;   // fp16 copy: 1/sqrt(a_ii), colour order
   float *sigma = nullptr;  // fp16 copy: sqrt(d_f/d_c) per injected row
   int *perm = nullptr;
@@ -189,6 +204,7 @@ struct Ctx {
   int64_t bytes = 0;
   bool ready = false;
   int prepared = 0;  // bit 1: fp32 copies, bit 2: fp16 copies
+  int col_format = kColsCompact;  // column indices the kernels read (ColFormat)
   hipEvent_t ev[8] = {};
 };
 

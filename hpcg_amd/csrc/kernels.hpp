@@ -2,9 +2,11 @@
 //
 // Matrix storage: 27-slot sliced ELL.  Slot s of a row is the stencil offset
 // (dx,dy,dz) = (s%3-1, s/3%3-1, s/9-1); a neighbour outside the grid is a
-// zero value whose column is the row itself.  Within a slice of 64 rows the
-// layout is slot-major, so one wavefront reads each slot of its 64 rows as one
-// contiguous 256-byte (index) and 512-byte (fp64 value) run.  Slot order is
+// zero value whose explicit column is the row itself.  Within a slice of 64
+// rows the layout is slot-major, so one wavefront reads each slot of its 64
+// rows as one contiguous 512-byte (fp64 value) run, plus a 256-byte run of
+// explicit indices or a 64-byte run of int8 deltas, or no per-entry index at
+// all (ColForm).  Slot order is
 // ascending column order, the order in which the reference sums a row
 // (ComputeSPMV_ref.cpp:67-68); the zero slots add exact zeros, so sums match
 // the reference bit for bit.  The whole library is compiled with
@@ -47,18 +49,18 @@ __device__ __forceinline__ int64_t tile_id() {
   return xcd * per + (xcd < rem ? xcd : rem) + r;
 }
 
-// Occupancy of the fp64 sparse kernels with explicit column indices (SpMV,
-// Gauss-Seidel steps, restriction): at most two waves per SIMD, so a wave
-// holds the column, value and gather loads of all its slots in flight (169
-// VGPRs) instead of the default's batches of about 9 at six waves per SIMD.
-// Measured on the HPCG sets: 3 % faster at 512^3 (the CG SpMV 9 %), 1.7 % at
-// 192^3; eight waves per SIMD (64 VGPRs) is 10 % slower.  The fp32 / fp16 and
-// compact-index kernels keep the default (slower with the limit), and so do
-// steps that read fewer than 8 slots (the backend pads every kernel under the
+// Occupancy of the fp64 sparse kernels (SpMV, Gauss-Seidel steps,
+// restriction): at most two waves per SIMD, so a wave holds the column,
+// value and gather loads of all its slots in flight (169 VGPRs) instead of
+// the default's batches of about 9 at six waves per SIMD.  Measured on the
+// HPCG sets with explicit indices: 3 % faster at 512^3 (the CG SpMV 9 %),
+// 1.7 % at 192^3; eight waves per SIMD (64 VGPRs) is 10 % slower.  The fp32 /
+// fp16 kernels keep the default (slower with the limit), and so do steps
+// that read fewer than 8 slots (the backend pads every kernel under the
 // limit to its register budget).
 template <class VT, bool CI = false, int SLOTS = 27>
 struct SparseOccupancy {
-  static constexpr unsigned max_waves = (sizeof(VT) == 8 && !CI && SLOTS >= 8) ? 2 : 8;
+  static constexpr unsigned max_waves = (sizeof(VT) == 8 && SLOTS >= 8) ? 2 : 8;
 };
 #define HPCGX_SPARSE_OCCUPANCY(VT, CI) __attribute__((amdgpu_waves_per_eu(1, SparseOccupancy<VT, CI>::max_waves)))
 
@@ -90,14 +92,21 @@ __device__ __forceinline__ ACC ld_val(const __half *p) {
   return (ACC)__half2float(__ushort_as_half(bits));
 }
 
-// Column index of (row, slot): explicit 32-bit index, or for the compact
-// copies base + lane + int8 delta with a per-slice fallback.
+// Column index of (row, slot): the explicit 32-bit index, or the compact
+// form (ColForm) with a per-slice base.
 struct ColIdx {
   const int *cols;
   const int *cbase;     // null: explicit indices only
   const int8_t *cdelta;
-  const uint8_t *cwide;
+  const uint8_t *cform;
 };
+
+// The columns a launch reads on a level: the compact form when the context
+// uses it and the level has it, else the explicit indices.
+inline ColIdx col_idx(const Ctx *c, const Level &L) {
+  if (c->col_format == kColsCompact && L.cbase) return ColIdx{L.cols, L.cbase, L.cdelta, L.cform};
+  return ColIdx{L.cols, nullptr, nullptr, nullptr};
+}
 
 // Slice of a row.  UNI: the 64 lanes of the wave hold the 64 rows of one
 // slice (launches over 64-aligned rows), so the slice is wave-uniform and the
@@ -108,32 +117,41 @@ __device__ __forceinline__ int slice_of(int64_t row) {
   else return (int)(row >> 6);
 }
 
-// Column of (row, slot s).  CM 0: the explicit 32-bit index; CM 1: the
-// compact form base + lane + int8 delta.  A launch picks one per wave
-// (slice_wide), so each inner loop is straight-line code.
+// Column of (row, slot s).  CM 0: the explicit 32-bit index; CM 1: base +
+// lane + int8 delta; CM 2: base + lane.  A launch picks one per slice
+// (with_cols), so each inner loop is straight-line code.
 template <int CM, bool NT>
 __device__ __forceinline__ int col_at(const ColIdx &ci, int64_t row, int slice, size_t b, int s) {
   if constexpr (CM == 0) {
     return ld_col<NT>(ci.cols + b + s * kSlice);
-  } else {
+  } else if constexpr (CM == 1) {
     const int base = ci.cbase[(int64_t)slice * kSlots + s];
     int8_t d;
     if constexpr (NT) d = __builtin_nontemporal_load(ci.cdelta + b + s * kSlice);
     else d = ci.cdelta[b + s * kSlice];
     return base + (int)(row & 63) + (int)d;
+  } else {
+    return ci.cbase[(int64_t)slice * kSlots + s] + (int)(row & 63);
   }
 }
 
 template <int V>
 using cmode = std::integral_constant<int, V>;
 
-// Whether the slice keeps explicit indices; the same for the whole wave when
-// the slice is (UNI), which keeps the choice a scalar branch.
-template <bool CI, bool UNI = true>
-__device__ __forceinline__ bool slice_wide(const ColIdx &ci, int slice) {
-  if constexpr (!CI) return false;
-  else if constexpr (UNI) return __builtin_amdgcn_readfirstlane((int)ci.cwide[slice]) != 0;
-  else return ci.cwide[slice] != 0;
+// Runs f(cmode<CM>{}) with the column mode of the row's slice.  CI: the
+// launch reads compact indices.  UNI: the slice is the same for the whole
+// wave, which keeps the choice a scalar branch.
+template <bool CI, bool UNI, class F>
+__device__ __forceinline__ auto with_cols(const ColIdx &ci, int slice, F &&f) {
+  if constexpr (!CI) {
+    return f(cmode<0>{});
+  } else {
+    int form = ci.cform[slice];
+    if constexpr (UNI) form = __builtin_amdgcn_readfirstlane(form);
+    if (form == kFormUniform) return f(cmode<2>{});
+    if (form == kFormDelta) return f(cmode<1>{});
+    return f(cmode<0>{});
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -287,23 +305,27 @@ __device__ __forceinline__ int64_t chunk_index(int j) {
 // ---------------------------------------------------------------------------
 // sparse kernels (one thread per row, 4 slices per block)
 
-// y = A x over rows [row0, row0 + nrows); ComputeSPMV_ref.cpp:61-70
-template <class VT, class XT, class ACC>
-__global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, false) void k_spmv(const int *__restrict__ cols,
+// y = A x over rows [0, nrows); ComputeSPMV_ref.cpp:61-70
+template <class VT, class XT, class ACC, bool CI>
+__global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_spmv(const ColIdx ci,
                                                  const VT *__restrict__ vals,
                                                  const XT *__restrict__ x, XT *__restrict__ y,
                                                  int64_t nrows) {
   const int64_t row = tile_id() * kBlock + threadIdx.x;
   if (row >= nrows) return;
   const size_t b = ell_base(row);
-  ACC sum = 0;
+  const int slice = slice_of<true>(row);
+  y[row] = (XT)with_cols<CI, true>(ci, slice, [&](auto cm) {
+    constexpr int CM = decltype(cm)::value;
+    ACC sum = 0;
 #pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    const int c = ld_col(cols + b + s * kSlice);
-    const ACC v = ld_val<ACC>(vals + b + s * kSlice);
-    sum += v * (ACC)x[c];
-  }
-  y[row] = (XT)sum;
+    for (int s = 0; s < kSlots; ++s) {
+      const int c = col_at<CM, true>(ci, row, slice, b, s);
+      const ACC v = ld_val<ACC>(vals + b + s * kSlice);
+      sum += v * (ACC)x[c];
+    }
+    return sum;
+  });
 }
 
 // Slots whose neighbour lies in a block swept before block k in the default
@@ -361,13 +383,7 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
     }
     return sum;
   };
-  ACC sum;
-  if constexpr (CI) {
-    if (slice_wide<CI>(ci, slice)) sum = row_sum(cmode<0>{});
-    else sum = row_sum(cmode<1>{});
-  } else {
-    sum = row_sum(cmode<0>{});
-  }
+  ACC sum = with_cols<CI, true>(ci, slice, row_sum);
   if (ZK < 0) sum += (ACC)x[row] * d;
   const XT xn = (XT)(sum / d);
   x[row] = xn;
@@ -430,16 +446,12 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_restr
     return ax;
   };
   ACC ax;
-  if constexpr (CI) {
-    if (frow) {  // rows from a list: slices differ across the wave
-      const int slice = slice_of<false>(row);
-      ax = slice_wide<CI, false>(ci, slice) ? row_ax(cmode<0>{}, slice) : row_ax(cmode<1>{}, slice);
-    } else {
-      const int slice = slice_of<true>(row);
-      ax = slice_wide<CI, true>(ci, slice) ? row_ax(cmode<0>{}, slice) : row_ax(cmode<1>{}, slice);
-    }
+  if (frow) {  // rows from a list: slices differ across the wave
+    const int slice = slice_of<false>(row);
+    ax = with_cols<CI, false>(ci, slice, [&](auto cm) { return row_ax(cm, slice); });
   } else {
-    ax = row_ax(cmode<0>{}, 0);
+    const int slice = slice_of<true>(row);
+    ax = with_cols<CI, true>(ci, slice, [&](auto cm) { return row_ax(cm, slice); });
   }
   ACC res = (ACC)r[row] - ax;
   if (SCALE) res = res * (ACC)sigma[t];

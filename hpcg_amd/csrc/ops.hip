@@ -18,33 +18,27 @@ const float *vals_of<float>(const Level &L) { return L.vals32; }
 template <>
 const __half *vals_of<__half>(const Level &L) { return L.vals16; }
 
-// Column indices of a level's copy in precision VT: the reduced-precision
-// copies use the compact form when it was built.
-template <class VT>
-static ColIdx col_idx(const Level &L) {
-  if (sizeof(VT) < 8 && L.cbase) return ColIdx{L.cols, L.cbase, L.cdelta, L.cwide};
-  return ColIdx{L.cols, nullptr, nullptr, nullptr};
-}
-
 // rz (fp64 stencil levels only): the step also leaves the partials of r.x
 // over its rows there, tiles(rows) of them.
 template <class VT, class XT, class ACC, int ZK, bool FINE, bool SAFE = false>
 static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, int k, double *rz = nullptr) {
   const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
   if (nr <= 0) return;
-  const ColIdx ci = col_idx<VT>(L);
-  // Coarse fp64 steps with explicit indices run at two waves per SIMD (see
-  // HPCGX_SPARSE_OCCUPANCY); launched as one-wave blocks, those of blocks up
-  // to 2^20 rows spread over more CUs: a 192^3 set measured 2-6 % faster,
-  // 512^3 about 5 %.  The fine level, the other precisions and the r.z form
-  // keep 256-thread blocks.
-  const unsigned bs = (!FINE && sizeof(VT) == 8 && !ci.cbase && nr < ((int64_t)1 << 20)) ? kSlice : kBlock;
+  const ColIdx ci = col_idx(c, L);
+  // Coarse fp64 steps run at two waves per SIMD (see HPCGX_SPARSE_OCCUPANCY);
+  // launched as one-wave blocks, those of blocks up to 2^20 rows spread over
+  // more CUs: a 192^3 set measured 2-6 % faster, 512^3 about 5 %.  The fine
+  // level, the other precisions and the r.z form keep 256-thread blocks.
+  const unsigned bs = (!FINE && sizeof(VT) == 8 && nr < ((int64_t)1 << 20)) ? kSlice : kBlock;
   const unsigned g = (unsigned)((nr + bs - 1) / bs);
   hipStream_t st = c->stream;
   if constexpr (ZK < 0 && sizeof(VT) == 8 && sizeof(XT) == 8) {
     if (rz) {
       // 256-thread blocks: the partials are per block of 4 waves, tiles(nr) of them
-      k_gs<VT, XT, ACC, -1, FINE, false, false, true><<<tiles(nr), kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr, rz);
+      if (ci.cbase)
+        k_gs<VT, XT, ACC, -1, FINE, true, false, true><<<tiles(nr), kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr, rz);
+      else
+        k_gs<VT, XT, ACC, -1, FINE, false, false, true><<<tiles(nr), kBlock, 0, st>>>(ci, v, r, x, r0, nr, nullptr, rz);
       return;
     }
   }
@@ -162,7 +156,7 @@ static int vcycle(Ctx *c, int l, const XT *r, XT *x, double *rz = nullptr, int64
     xc_v = reinterpret_cast<XT *>(C.x32);
   }
   const int64_t nc = L.ninj, r0 = L.lay.start[L.lay.inj];
-  const ColIdx ci = col_idx<VT>(L);
+  const ColIdx ci = col_idx(c, L);
   if (ci.cbase)
     k_restrict<VT, XT, ACC, SCALE, true><<<tiles(nc), kBlock, 0, c->stream>>>(ci, vals_of<VT>(L), r, x, rc_v,
                                                                              L.inj, L.sigma, r0, nc, L.frow);
@@ -295,8 +289,11 @@ __global__ __launch_bounds__(kBlock) void k_to_double(const float *__restrict__ 
 
 int spmv_fp64(Ctx *c, int level, const double *x, double *y) {
   const Level &L = c->levels[level];
-  k_spmv<double, double, double><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(L.cols, L.vals, x, y,
-                                                                              L.lay.npad);
+  const ColIdx ci = col_idx(c, L);
+  if (ci.cbase)
+    k_spmv<double, double, double, true><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(ci, L.vals, x, y, L.lay.npad);
+  else
+    k_spmv<double, double, double, false><<<tiles(L.lay.npad), kBlock, 0, c->stream>>>(ci, L.vals, x, y, L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }
@@ -356,9 +353,13 @@ int hpcgx_restrict(hpcgx_ctx *c, int level, const double *r, const double *x, do
     return rc;
   const Level &L = c->levels[level];
   const int64_t nc = L.ninj;
-  k_restrict<double, double, double, false, false><<<tiles(nc), kBlock, 0, c->stream>>>(
-      ColIdx{L.cols, nullptr, nullptr, nullptr}, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[L.lay.inj], nc,
-      L.frow);
+  const ColIdx ci = col_idx(c, L);
+  if (ci.cbase)
+    k_restrict<double, double, double, false, true><<<tiles(nc), kBlock, 0, c->stream>>>(
+        ci, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[L.lay.inj], nc, L.frow);
+  else
+    k_restrict<double, double, double, false, false><<<tiles(nc), kBlock, 0, c->stream>>>(
+        ci, L.vals, r, x, rc_out, L.inj, nullptr, L.lay.start[L.lay.inj], nc, L.frow);
   HPCGX_CHECK_LAUNCH(c);
   return HPCGX_OK;
 }

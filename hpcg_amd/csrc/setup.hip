@@ -52,8 +52,9 @@ void dfree(Ctx *c, void *p, size_t bytes) {
   c->bytes -= (int64_t)(bytes ? bytes : 8);
 }
 
+// Runs f(lo, hi) over up to 16 contiguous ranges of [0, n) on host threads.
 template <class F>
-static void parallel_for(int64_t n, F f) {
+static void parallel_ranges(int64_t n, F f) {
   unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   if (n < 65536) nt = 1;
   std::vector<std::thread> th;
@@ -61,11 +62,16 @@ static void parallel_for(int64_t n, F f) {
   for (unsigned t = 0; t < nt; ++t) {
     const int64_t lo = t * chunk, hi = std::min<int64_t>(n, lo + chunk);
     if (lo >= hi) break;
-    th.emplace_back([=] {
-      for (int64_t i = lo; i < hi; ++i) f(i);
-    });
+    th.emplace_back([=] { f(lo, hi); });
   }
   for (auto &t : th) t.join();
+}
+
+template <class F>
+static void parallel_for(int64_t n, F f) {
+  parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) f(i);
+  });
 }
 
 Layout make_layout(int nx, int ny, int nz, const int *order) {
@@ -173,7 +179,7 @@ static void free_level(Ctx *c, Level &L) {
   dfree(c, L.vals16, m * sizeof(__half));
   dfree(c, L.cbase, (size_t)(L.lay.npad / kSlice) * kSlots * sizeof(int));
   dfree(c, L.cdelta, m * sizeof(int8_t));
-  dfree(c, L.cwide, (size_t)(L.lay.npad / kSlice));
+  dfree(c, L.cform, (size_t)(L.lay.npad / kSlice));
   dfree(c, L.dinv, L.lay.npad * sizeof(float));
   dfree(c, L.sigma, L.ninj * sizeof(float));
   dfree(c, L.perm, L.lay.n * sizeof(int));
@@ -404,14 +410,18 @@ static int pack_stencil_device(Ctx *c, Level &L, RowAcc rows, bool *stencil) {
     const int64_t i0 = q * kUpRows, m = std::min<int64_t>(kUpRows, n - i0);
     if (q >= 2) HPCGX_HIP(c, hipEventSynchronize(c->up_ev[k]));  // host buffer free again
     UpChunk h = up_view(c->up_host[k]), d = up_view(c->up_dev[k]);
-    parallel_for(m, [&](int64_t t) {
-      const int64_t i = i0 + t;
-      int cnt = rows.nnz(i);
-      if (cnt < 1 || cnt > kSlots) cnt = 0;  // row length 0: rejected on the device
-      h.nr[t] = (uint8_t)cnt;
-      std::memcpy(h.cols + t * kSlots, rows.cols(i), cnt * sizeof(int));
-      std::memcpy(h.vals + t * kSlots, rows.vals(i), cnt * sizeof(double));
-      nnz += cnt;
+    parallel_ranges(m, [&](int64_t lo, int64_t hi) {
+      int64_t sum = 0;  // one atomic add per range: per-row adds contend on one line
+      for (int64_t t = lo; t < hi; ++t) {
+        const int64_t i = i0 + t;
+        int cnt = rows.nnz(i);
+        if (cnt < 1 || cnt > kSlots) cnt = 0;  // row length 0: rejected on the device
+        h.nr[t] = (uint8_t)cnt;
+        std::memcpy(h.cols + t * kSlots, rows.cols(i), cnt * sizeof(int));
+        std::memcpy(h.vals + t * kSlots, rows.vals(i), cnt * sizeof(double));
+        sum += cnt;
+      }
+      nnz += sum;
     });
     HPCGX_HIP(c, hipMemcpyAsync(d.vals, h.vals, (size_t)m * kSlots * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HPCGX_HIP(c, hipMemcpyAsync(d.cols, h.cols, (size_t)m * kSlots * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -628,6 +638,31 @@ static void order_colours(int64_t n, int ncol, std::vector<int> &colour, const i
   for (auto &cl : colour) cl = rank[cl];
 }
 
+// One padded row p of a general level in the ELL arrays; returns its entries.
+template <class RowAcc>
+static int pack_general_row(Level &L, RowAcc &rows, int64_t p, std::vector<int> &hc, std::vector<double> &hv,
+                            std::vector<uint8_t> &hd) {
+  const size_t b = (size_t)(p >> 6) * (kSlots * kSlice) + (size_t)(p & 63);
+  const int i = L.iperm_h[p];
+  for (int s = 0; s < kSlots; ++s) {
+    hc[b + s * kSlice] = (int)p;
+    hv[b + s * kSlice] = (i < 0 && s == 0) ? 1.0 : 0.0;
+  }
+  if (i < 0) return 0;
+  const int cnt = rows.nnz(i);
+  const int *rc = rows.cols(i);
+  const double *rv = rows.vals(i);
+  int ds = -1;
+  for (int j = 0; j < cnt; ++j) {
+    hc[b + j * kSlice] = L.perm_h[rc[j]];
+    hv[b + j * kSlice] = rv[j];
+    if (rc[j] == i && ds < 0) ds = j;
+  }
+  hd[p] = (uint8_t)ds;
+  L.rowlen_h[i] = (uint8_t)cnt;
+  return cnt;
+}
+
 // General layout for arbitrary rows (the reference requires every
 // optimisation to work for general sparse matrices, GenerateProblem.cpp:47-49).
 // Colouring: first fit, like the reference's greedy example
@@ -694,26 +729,10 @@ static int pack_general(Ctx *c, Level &L, RowAcc rows, const int *f2c, int64_t n
   hd.assign(npad, 0);
   L.rowlen_h.assign(n, 0);
   std::atomic<int64_t> nnz(0);
-  parallel_for(npad, [&](int64_t p) {
-    const size_t b = (size_t)(p >> 6) * (kSlots * kSlice) + (size_t)(p & 63);
-    const int i = L.iperm_h[p];
-    for (int s = 0; s < kSlots; ++s) {
-      hc[b + s * kSlice] = (int)p;
-      hv[b + s * kSlice] = (i < 0 && s == 0) ? 1.0 : 0.0;
-    }
-    if (i < 0) return;
-    const int cnt = rows.nnz(i);
-    const int *rc = rows.cols(i);
-    const double *rv = rows.vals(i);
-    int ds = -1;
-    for (int j = 0; j < cnt; ++j) {
-      hc[b + j * kSlice] = L.perm_h[rc[j]];
-      hv[b + j * kSlice] = rv[j];
-      if (rc[j] == i && ds < 0) ds = j;
-    }
-    hd[p] = (uint8_t)ds;
-    L.rowlen_h[i] = (uint8_t)cnt;
-    nnz += cnt;
+  parallel_ranges(npad, [&](int64_t lo, int64_t hi) {
+    int64_t sum = 0;
+    for (int64_t p = lo; p < hi; ++p) sum += pack_general_row(L, rows, p, hc, hv, hd);
+    nnz += sum;
   });
   L.nnz = nnz;
   return HPCGX_OK;
@@ -795,6 +814,8 @@ struct PointerRows {
 
 // Builds what the V-cycle needs once every level is present: the injection
 // maps, the coarse work vectors and the solver buffers.
+static int build_compact_all(Ctx *c);
+
 int ensure_ready(Ctx *c) {
   if (c->ready) return HPCGX_OK;
   if (c->levels.empty()) return set_err(c, HPCGX_ESTATE, "no problem: upload or generate levels first");
@@ -859,6 +880,7 @@ int ensure_ready(Ctx *c) {
       if ((rc = dalloc(c, (void **)b, np * sizeof(double)))) return rc;
     c->cg_len = np;
   }
+  if ((rc = build_compact_all(c))) return rc;
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   c->ready = true;
   return HPCGX_OK;
@@ -926,20 +948,24 @@ __global__ void k_sigma(const double *__restrict__ fvals, const uint8_t *__restr
 // value) are base + lane + d with d in [-128, 127]; a zero-valued slot
 // (neighbour outside the grid, unused entry) may name any row of the level,
 // as its product is an exact zero: it takes its own row when reachable, else
-// the nearest reachable one.  A slice whose columns do not fit is flagged
-// wide and keeps its explicit indices.
+// the nearest reachable one.  A slice whose columns do not fit is wide and
+// keeps its explicit indices.  A slice in which every slot's stored entries
+// share one offset (column - lane), and whose zero-valued slots may then
+// take base + lane (a row of the level), is uniform: base + lane is its
+// column, with no per-entry index.  The form depends only on the columns and
+// on which entries are zero, never on the matrix being a stencil.
 __global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__ cols,
                                                          const double *__restrict__ vals,
                                                          const uint8_t *__restrict__ dslot,
                                                          int *__restrict__ cbase, int8_t *__restrict__ cdelta,
-                                                         uint8_t *__restrict__ cwide, int64_t nslices, int64_t npad) {
+                                                         uint8_t *__restrict__ cform, int64_t nslices, int64_t npad) {
   const int lane = threadIdx.x & 63;
   const int64_t slice = (int64_t)blockIdx.x * (kBlock / kSlice) + (threadIdx.x >> 6);
   if (slice >= nslices) return;
   const int64_t row = slice * kSlice + lane;
   const size_t b = (size_t)slice * (kSlots * kSlice) + lane;
   const int ds = dslot ? dslot[row] : kDiagSlot;
-  bool wide = false;
+  bool wide = false, uniform = true;
   for (int s = 0; s < kSlots; ++s) {
     const int col = cols[b + s * kSlice];
     const bool live = vals[b + s * kSlice] != 0.0 || s == ds;
@@ -953,8 +979,11 @@ __global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__
       base = slice * kSlice;  // no live lane: every lane names its own row
     } else {
       if ((int64_t)hi - lo > 255) wide = true;
+      if (hi != lo) uniform = false;
       base = (int64_t)lo + (((int64_t)hi - lo + 1) >> 1);
     }
+    // uniform: every lane reads base + lane, which must be a row of the level
+    if (base + lane < 0 || base + lane >= npad) uniform = false;
     int64_t target = live ? (int64_t)col : row;
     const int64_t rlo = max<int64_t>(0, base + lane - 128), rhi = min<int64_t>(npad - 1, base + lane + 127);
     if (rlo > rhi) wide = true;
@@ -964,9 +993,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__
     cdelta[b + s * kSlice] = (int8_t)(wide ? 0 : d);
     if (lane == 0) cbase[slice * kSlots + s] = (int)base;
   }
-  // wide is per lane so far; the slice is wide if any lane is
-  const bool any = __any(wide);
-  if (lane == 0) cwide[slice] = any ? 1 : 0;
+  // wide and uniform are per lane so far; the slice is wide if any lane is,
+  // uniform if every lane is
+  const bool any_wide = __any(wide), all_uniform = __all(uniform);
+  if (lane == 0) cform[slice] = any_wide ? kFormWide : all_uniform ? kFormUniform : kFormDelta;
 }
 
 static int build_compact_cols(Ctx *c, Level &L) {
@@ -975,11 +1005,21 @@ static int build_compact_cols(Ctx *c, Level &L) {
   int rc;
   if ((rc = dalloc(c, (void **)&L.cbase, (size_t)ns * kSlots * sizeof(int), false))) return rc;
   if ((rc = dalloc(c, (void **)&L.cdelta, (size_t)L.lay.npad * kSlots * sizeof(int8_t), false))) return rc;
-  if ((rc = dalloc(c, (void **)&L.cwide, (size_t)ns, false))) return rc;
+  if ((rc = dalloc(c, (void **)&L.cform, (size_t)ns, false))) return rc;
   const int64_t grid = (ns + (kBlock / kSlice) - 1) / (kBlock / kSlice);
-  k_compact_cols<<<grid, kBlock, 0, c->stream>>>(L.cols, L.vals, L.dslot, L.cbase, L.cdelta, L.cwide, ns,
+  k_compact_cols<<<grid, kBlock, 0, c->stream>>>(L.cols, L.vals, L.dslot, L.cbase, L.cdelta, L.cform, ns,
                                                  L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
+  return HPCGX_OK;
+}
+
+// The compact index copies of every level (the default column format).
+static int build_compact_all(Ctx *c) {
+  if (c->col_format != kColsCompact) return HPCGX_OK;
+  for (auto &L : c->levels) {
+    int rc = build_compact_cols(c, L);
+    if (rc) return rc;
+  }
   return HPCGX_OK;
 }
 
@@ -990,7 +1030,6 @@ static int build_precision(Ctx *c, int prec) {
   for (int l = 0; l < nl; ++l) {
     Level &L = c->levels[l];
     const int64_t m = L.lay.npad * kSlots;
-    if ((rc = build_compact_cols(c, L))) return rc;
     if (prec == HPCGX_PREC_FP32) {
       if (!L.vals32 && (rc = dalloc(c, (void **)&L.vals32, m * sizeof(float), false))) return rc;
       k_to_f32<<<nblk(m), kBlock, 0, c->stream>>>(L.vals, L.vals32, m);
@@ -1326,6 +1365,35 @@ int hpcgx_prepare_precision(hpcgx_ctx *c, int precision) {
   HPCGX_HIP(c, hipSetDevice(c->device));
   if (precision == HPCGX_PREC_FP64) return ensure_ready(c);
   return build_precision(c, precision);
+}
+
+int hpcgx_set_column_format(hpcgx_ctx *c, int format) {
+  if (!c) return HPCGX_EINVAL;
+  if (format != HPCGX_COLS_EXPLICIT && format != HPCGX_COLS_COMPACT)
+    return set_err(c, HPCGX_EINVAL, "set_column_format: HPCGX_COLS_EXPLICIT or HPCGX_COLS_COMPACT");
+  HPCGX_HIP(c, hipSetDevice(c->device));
+  c->col_format = format == HPCGX_COLS_COMPACT ? kColsCompact : kColsExplicit;
+  if (c->ready) {
+    int rc = build_compact_all(c);
+    if (rc) return rc;
+    HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return HPCGX_OK;
+}
+
+int hpcgx_level_column_forms(const hpcgx_ctx *c, int level, int64_t *counts) {
+  if (!c || !counts) return HPCGX_EINVAL;
+  if (level < 0 || level >= (int)c->levels.size()) return HPCGX_EINVAL;
+  const Level &L = c->levels[level];
+  counts[0] = counts[1] = counts[2] = 0;
+  if (!L.cbase) return HPCGX_OK;
+  const int64_t ns = L.lay.npad / kSlice;
+  std::vector<uint8_t> f(ns);
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+      hipMemcpy(f.data(), L.cform, ns, hipMemcpyDeviceToHost) != hipSuccess)
+    return HPCGX_EHIP;
+  for (const uint8_t v : f) counts[v < 3 ? v : 1] += 1;
+  return HPCGX_OK;
 }
 
 double hpcgx_memory_bytes(const hpcgx_ctx *c) { return c ? (double)c->bytes : 0.0; }

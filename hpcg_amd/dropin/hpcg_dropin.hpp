@@ -211,6 +211,9 @@ typedef struct SparseMatrix_STRUCT SparseMatrix;
 extern "C" {
 void HPCG_MatrixDiagonalReplaced(SparseMatrix *A, const double *diag);
 void HPCG_MatrixRelease(SparseMatrix *A);
+// Frees the matrix arrays if the drop-in's GenerateProblem allocated them
+// (one block per array) and returns 1; returns 0 for any other matrix.
+int HPCG_MatrixStorageRelease(SparseMatrix *A);
 }
 
 inline void InitializeSparseMatrix(SparseMatrix &A, Geometry *geom) {
@@ -247,12 +250,26 @@ inline void ReplaceMatrixDiagonal(SparseMatrix &A, Vector &diagonal) {
   if (A.optimizationData) HPCG_MatrixDiagonalReplaced(&A, diagonal.values);
 }
 
+// Frees the matrix as SparseMatrix.hpp:151-183 does.  Its arrays are either
+// the drop-in GenerateProblem's blocks (freed by HPCG_MatrixStorageRelease),
+// one allocation per array (HPCG_CONTIGUOUS_ARRAYS, GenerateProblem_ref.cpp:116-124)
+// or one allocation per row (GenerateProblem_ref.cpp:105-112).
 inline void DeleteMatrix(SparseMatrix &A) {
   if (A.optimizationData) HPCG_MatrixRelease(&A);
-  for (local_int_t i = 0; i < A.localNumberOfRows; ++i) {
-    delete[] A.matrixValues[i];
-    delete[] A.mtxIndG[i];
-    delete[] A.mtxIndL[i];
+  if (!HPCG_MatrixStorageRelease(&A) && A.matrixValues) {
+#ifdef HPCG_CONTIGUOUS_ARRAYS
+    if (A.localNumberOfRows > 0) {
+      delete[] A.matrixValues[0];
+      delete[] A.mtxIndG[0];
+      delete[] A.mtxIndL[0];
+    }
+#else
+    for (local_int_t i = 0; i < A.localNumberOfRows; ++i) {
+      delete[] A.matrixValues[i];
+      delete[] A.mtxIndG[i];
+      delete[] A.mtxIndL[i];
+    }
+#endif
   }
   if (A.title) delete[] A.title;
   if (A.nonzerosInRow) delete[] A.nonzerosInRow;
@@ -317,5 +334,7 @@ int ComputeWAXPBY(const local_int_t n, const double alpha, const Vector &x, cons
 int OptimizeProblem(SparseMatrix &A, CGData &data, Vector &b, Vector &x, Vector &xexact);  // OptimizeProblem.hpp:22
 double OptimizeProblemMemoryUse(const SparseMatrix &A);                          // OptimizeProblem.hpp:28
 void ExchangeHalo(const SparseMatrix &A, Vector &x);                              // ExchangeHalo.hpp:19
+void GenerateProblem(SparseMatrix &A, Vector *b, Vector *x, Vector *xexact);      // GenerateProblem.hpp:19
+void SetupHalo(SparseMatrix &A);                                                  // SetupHalo.hpp:19
 
 #endif  // HPCG_DROPIN_HPP

@@ -129,3 +129,70 @@ def hpcg_rhs(nx, ny, nz):
         return c
     nnz = cnt(nz)[:, None, None] * cnt(ny)[None, :, None] * cnt(nx)[None, None, :]
     return (26.0 - (nnz - 1.0)).reshape(-1)
+
+
+DEFAULT_ORDER = (7, 3, 4, 5, 2, 6, 1, 0)  # parity class swept k-th forward (the library's default)
+
+
+def zero_start_slots(order=DEFAULT_ORDER):
+    """Slots a forward step from x = 0 reads for colour block k (27-point
+    stencil, blocks in `order`): the neighbours whose class was swept
+    before, i.e. whose x is no longer zero.  Block 0 reads none."""
+    pos = {c: k for k, c in enumerate(order)}
+    out = []
+    for k, c in enumerate(order):
+        cnt = 0
+        for s in range(27):
+            odd = int(s % 3 != 1) | (int((s // 3) % 3 != 1) << 1) | (int(s // 9 != 1) << 2)
+            if pos[c ^ odd] < k:
+                cnt += 1
+        out.append(cnt)
+    return out
+
+
+def index_bytes_per_slot(forms):
+    """Column-index bytes a kernel loads per ELL slot, from a level's slice
+    counts per column form ({'uniform', 'delta', 'explicit'}, hpcgx_level_column_forms;
+    None: explicit 32-bit indices): 0 / 1 / 4 bytes per slot, plus the
+    per-slice 32-bit base of each slot (4/64 byte per slot) for the compact
+    forms."""
+    if forms is None:
+        return 4.0
+    tot = float(sum(forms.values()))
+    if tot == 0:
+        return 4.0
+    fu, fd, fw = forms["uniform"] / tot, forms["delta"] / tot, forms["explicit"] / tot
+    return fd * 1.0 + fw * 4.0 + (fu + fd) * 4.0 / 64.0
+
+
+def implementation_bytes_per_iteration(nx, ny, nz, levels=4, v=8, s=8, forms=None, order=DEFAULT_ORDER):
+    """Bytes the FP64 iteration's kernels move as built, in the convention
+    of algorithmic_bytes_per_iteration (each array element once per pass,
+    neighbour gathers not counted), but counting matrix slots as loaded: all
+    27 slots of a row (the zeros of rows at the grid boundary included), only
+    the slots of blocks swept before in the zero-start forward sweep (plus
+    the diagonal), 15 colour steps per symmetric sweep (the last block's
+    backward step is elided), column-index bytes per the level's column
+    forms (forms[l], see index_bytes_per_slot), no separate r.z pass (its
+    partials come from the post-smoother).  Even grids (every colour block
+    holds n/8 rows)."""
+    shapes = level_shapes(nx, ny, nz, levels)
+    zs = zero_start_slots(order)
+    total = 0.0
+    for l, (n, m, d) in enumerate(shapes):
+        ib = index_bytes_per_slot(None if forms is None else forms[l])
+        nb = n / 8.0
+        full = 27 * (v + ib) + 3 * s
+        # zero-start forward sweep: block k reads zs[k] neighbour slots, the diagonal, r; writes x
+        pre = sum(nb * (zs[k] * (v + ib) + v + 2 * s) for k in range(8)) + 7 * nb * full
+        total += pre
+        if l + 1 < levels:
+            nc = shapes[l + 1][0]
+            total += 15 * nb * full                                   # post-smoother
+            total += nb * 27 * (v + ib) + n * s + nc * (4 + 2 * s)    # residual + restriction
+            total += nc * (4 + 3 * s)                                 # prolongation
+    n0 = shapes[0][0]
+    ib0 = index_bytes_per_slot(None if forms is None else forms[0])
+    total += n0 * 27 * (8 + ib0) + 2 * n0 * 8                         # CG SpMV + p.Ap
+    total += 24 * n0 + 48 * n0                                         # p update; x, r update + r.r
+    return total

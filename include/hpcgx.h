@@ -43,6 +43,10 @@ extern "C" {
 #define HPCGX_PREC_FP32 1
 #define HPCGX_PREC_FP16 2 /* values D^-1/2 A D^-1/2 in fp16, fp32 arithmetic */
 
+/* column index formats (hpcgx_set_column_format) */
+#define HPCGX_COLS_EXPLICIT 0 /* 32-bit column per stored entry, the reference's mtxIndL */
+#define HPCGX_COLS_COMPACT 1  /* per 64-row slice: base + lane, + an int8 delta where needed */
+
 /* hpcgx_cg variants */
 #define HPCGX_CG_FP64 0       /* CG.cpp recurrence, fp64 MG */
 #define HPCGX_CG_MIXED_FP32 1 /* flexible PCG, fp64 outer, fp32 MG */
@@ -131,6 +135,20 @@ int hpcgx_level_get_diagonal(hpcgx_ctx *ctx, int level, double *diag);
  * packed like hpcgx_level_upload_packed (unused slots: col -1, value 0); the
  * inverse of the upload, for inspection and tests (SparseMatrix.hpp:47-51). */
 int hpcgx_level_export_packed(hpcgx_ctx *ctx, int level, uint8_t *nnz_row, int *cols, double *vals);
+/* Column indices the kernels read (OptimizeProblem.cpp:37 lets the matrix be
+ * stored in any form).  HPCGX_COLS_COMPACT (the default): per 64-row slice and
+ * slot one 32-bit base; a slice whose stored entries of each slot sit at base
+ * + lane reads no per-entry index, other slices read an int8 delta per entry,
+ * and slices whose deltas do not fit in int8 keep the explicit 32-bit
+ * indices.  Zero-valued entries may point anywhere in the level.  The form is
+ * chosen from the uploaded columns alone.  HPCGX_COLS_EXPLICIT: the 32-bit
+ * index of every entry (the reference's mtxIndL).  Results are the same bit
+ * for bit; only the bytes read differ. */
+int hpcgx_set_column_format(hpcgx_ctx *ctx, int format);
+/* Slices of a level per column form: counts[0] int8 deltas, counts[1]
+ * explicit indices, counts[2] no per-entry index (zeros before the compact
+ * copy exists). */
+int hpcgx_level_column_forms(const hpcgx_ctx *ctx, int level, int64_t *counts);
 /* Builds the fp32 or scaled-fp16 copies of every level for the mixed solvers. */
 int hpcgx_prepare_precision(hpcgx_ctx *ctx, int precision);
 /* Device bytes held by the context (OptimizeProblemMemoryUse, OptimizeProblem.cpp:103-107). */
