@@ -222,7 +222,7 @@ def tolerance_solves(ctx, db, dx, g, reps, tol=1e-9):
     import numpy as np
 
     import hpcg_amd
-    from hpcg_amd.models import algorithmic_bytes_per_iteration, hpcg_flops, hpcg_rhs, level_shapes
+    from hpcg_amd.models import algorithmic_bytes_per_iteration, hpcg_rhs, level_shapes, mxp_rate
     shapes = level_shapes(g, g, g)
     bn = float(np.linalg.norm(hpcg_rhs(g, g, g)))
     res = {}
@@ -236,7 +236,7 @@ def tolerance_solves(ctx, db, dx, g, reps, tol=1e-9):
         fwd = forward_error_ones(ctx, dx, shapes[0][0])
         per_it = o["seconds"] / max(1, o["niters"])
         by = algorithmic_bytes_per_iteration(g, g, g, v_mg=vmg, s_mg=smg)
-        mxp = hpcg_flops(shapes, o["niters"], sets=1)["total"] / o["seconds"] / 1e9
+        mxp = mxp_rate(shapes, o["niters"], o["seconds"])["gflops"]
         res[name] = {"seconds": round(o["seconds"], 4), "iterations": o["niters"],
                      "true_rel_residual": float(f"{rel:.4e}"), "forward_error": float(f"{fwd:.4e}"),
                      "ms_per_iteration": round(per_it * 1e3, 3),

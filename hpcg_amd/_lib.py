@@ -80,6 +80,16 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HpcgxError(f"{LIB_PATH} is not built: run `python -m hpcg_amd.build`")
+        # A process may hold only one HIP runtime.  torch ships its own
+        # libamdhip64 and reaches it under a file name the dynamic linker does
+        # not match against an already-loaded /opt/rocm copy, so if this
+        # library came first, torch would load a second runtime whose device
+        # discovery then fails ("No HIP GPUs are available").  Loaded after
+        # torch, libhpcgx.so binds to torch's runtime by its soname.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (rt, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -243,7 +243,7 @@ int zero_vec(Ctx *c, float *x, int64_t n);
 int ensure_reduce(Ctx *c, int64_t grid);
 // slot (F_NORMR2 only): record the norm there and run the deferred stop test
 int finish(Ctx *c, int nv, int64_t grid, int op, double *slot = nullptr);
-inline unsigned chunk_blocks(int64_t n) { return (unsigned)((n + 4 * 256 - 1) / (4 * 256)); }
+constexpr unsigned chunk_blocks(int64_t n) { return (unsigned)((n + 4 * 256 - 1) / (4 * 256)); }
 
 }  // namespace hpcgx
 

@@ -42,6 +42,16 @@ def hpcg_flops(shapes, iters, sets=1, ref_iters=None, pre=1, post=1):
     return dict(ddot=ddot, waxpby=waxpby, spmv=spmv, mg=mg, total=total)
 
 
+def mxp_rate(shapes, iters, seconds):
+    """HPCG-MxP-style rate of one mixed-precision solve: the operations the
+    HPCG model (ReportResults.cpp:81-99) credits for one set of `iters`
+    iterations (preamble included), counted as if done in FP64, over the
+    solve's wall time.  It is not an HPCG rating: the reference forbids
+    reduced precision, so it is reported apart from the FP64 rating."""
+    flops = hpcg_flops(shapes, iters, sets=1)["total"]
+    return {"iterations": int(iters), "flops": flops, "gflops": flops / seconds / 1e9}
+
+
 def colour_nnz(nx, ny, nz):
     """Stored nonzeros of the rows of each parity class c = (x&1)|(y&1)<<1|(z&1)<<2."""
     def per_dim(n):

@@ -53,6 +53,21 @@ def test_flop_model_matches_reference_closed_form():
     assert per_it / 1e9 == pytest.approx(2.618, abs=5e-4)
 
 
+def test_mxp_rate_counts_the_solves_iterations():
+    """The mixed solvers' HPCG-MxP-style rate credits the HPCG model's
+    operations for exactly the iterations the solve took (plus the set's
+    preamble: one dot, one WAXPBY, one SpMV), whatever their precision."""
+    from hpcg_amd.models import level_shapes, mxp_rate
+    sh = level_shapes(512, 512, 512)
+    n, m = sh[0][0], sh[0][1]
+    per_it = 12 * n + 2 * m + 10 * sum(s[1] for s in sh[:3]) + 4 * sh[3][1]
+    r = mxp_rate(sh, 232, 6.26)
+    assert r["iterations"] == 232
+    assert r["flops"] == pytest.approx(232 * per_it + 4 * n + 2 * m, rel=1e-15)
+    assert r["gflops"] == pytest.approx(r["flops"] / 6.26 / 1e9, rel=1e-15)
+    assert mxp_rate(sh, 233, 1.0)["flops"] - mxp_rate(sh, 232, 1.0)["flops"] == pytest.approx(per_it, rel=1e-12)
+
+
 def test_dropin_library_exports_reference_hooks():
     """The C++ hooks keep the reference signatures (CG.hpp:22-24,
     ComputeSPMV.hpp:20, ComputeSYMGS.hpp:20, ComputeMG.hpp:20,

@@ -43,6 +43,12 @@ def test_192_matrix_levels_bitexact(p192):
         assert np.array_equal(nr, onr), l
         assert np.array_equal(cols[mask], ocols[mask]), l
         assert np.array_equal(vals[mask], ovals[mask]), l
+    # compact columns: every slice but those at the ends of the level reads
+    # base + lane with no per-entry index
+    forms = ctx.column_forms(0)
+    assert sum(forms.values()) == ctx.info(0)["npad"] // 64, forms
+    assert forms["explicit"] == 0, forms
+    assert forms["uniform"] >= 0.999 * sum(forms.values()), forms
 
 
 def test_192_spmv_symgs_vcycle_bitexact(p192):

@@ -84,8 +84,13 @@ def test_compact_and_explicit_columns_agree(hip, case):
         forms = ctx.column_forms(0)
         assert sum(forms.values()) == ctx.info(0)["npad"] // 64
         if case.startswith("hpcg") or case.startswith("kappa"):
-            # stencil levels: nearly every slice needs no per-entry index
-            assert forms["uniform"] >= 0.9 * sum(forms.values()), forms
+            # stencil levels: no slice needs explicit indices, and most need
+            # no per-entry index.  The others are the slices whose base + lane
+            # would leave [0, npad) on a lane without a stored entry (their
+            # neighbours lie in the first or last rows of the level): 18 of
+            # the 64 slices at 16^3, a vanishing share on large grids
+            assert forms["explicit"] == 0, forms
+            assert forms["uniform"] >= 0.7 * sum(forms.values()), forms
     finally:
         ctx.close()
 
