@@ -62,6 +62,7 @@ SIGNATURES = {
     "hpcgx_waxpby": (C.c_int, [_vp, C.c_int64, C.c_double, _vp, C.c_double, _vp, _vp]),
     "hpcgx_waxpby_dot": (C.c_int, [_vp, C.c_int64, C.c_double, _vp, C.c_double, _vp, _vp,
                                    C.POINTER(C.c_double)]),
+    "hpcgx_dot_noalloc": (C.c_int, [_vp, C.c_int64, _vp, _vp, C.POINTER(C.c_double)]),
     "hpcgx_cg": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int,
                            C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double), _vp, _vp]),
     "hpcgx_kappa_grad": (C.c_int, [_vp, _vp, _vp, _vp]),
@@ -319,6 +320,14 @@ class Context:
         n = self.info(0)["n"] if n is None else n
         out = C.c_double()
         self._call("hpcgx_dot", self.h, n, _ptr(x), _ptr(y), C.byref(out))
+        return out.value
+
+    def dot_noalloc(self, x, y, n=None):
+        """hpcgx_dot_noalloc: the dot with its partial sums in the library's
+        static device scratch (allocates nothing)."""
+        n = self.info(0)["n"] if n is None else n
+        out = C.c_double()
+        self._call("hpcgx_dot_noalloc", self.h, n, _ptr(x), _ptr(y), C.byref(out))
         return out.value
 
     def waxpby(self, alpha, x, beta, y, w, n=None):

@@ -244,6 +244,8 @@ int ensure_reduce(Ctx *c, int64_t grid);
 // slot (F_NORMR2 only): record the norm there and run the deferred stop test
 int finish(Ctx *c, int nv, int64_t grid, int op, double *slot = nullptr);
 constexpr unsigned chunk_blocks(int64_t n) { return (unsigned)((n + 4 * 256 - 1) / (4 * 256)); }
+// Looks up the static scratch of hpcgx_dot_noalloc on `device` (current).
+int noalloc_scratch_init(int device);
 
 }  // namespace hpcgx
 

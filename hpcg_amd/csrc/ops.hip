@@ -2,6 +2,7 @@
 // restriction, prolongation, the V-cycle in fp64 / fp32 / scaled fp16, and
 // the vector reductions.
 #include <initializer_list>
+#include <mutex>
 
 #include "kernels.hpp"
 
@@ -302,6 +303,45 @@ int symgs_fp64(Ctx *c, int level, const double *r, double *x) {
   return symgs_launch<double, double, double>(c, level, r, x, false);
 }
 
+// ---------------------------------------------------------------------------
+// Static scratch of hpcgx_dot_noalloc, the dot product for callers that may
+// not allocate device memory.  It holds the partial sums of the producer
+// blocks of a level of up to kNoAllocRows padded rows (one per 1024 rows),
+// their folded sums and the finishing kernel's scalars: 2.1 MB, which each
+// device keeps until the process ends.  Every call on one device shares it.
+// Each call overwrites all it reads (every producer block writes its
+// partial, the fold every folded sum, the finishing kernel the result), so
+// nothing needs clearing.
+constexpr int64_t kNoAllocRows = (int64_t)1 << 28;
+constexpr int64_t kNaBlocks = kNoAllocRows / (kChunk * kBlock);
+constexpr int64_t kNaFolded = kNaBlocks;                                    // offset of the folded sums
+constexpr int64_t kNaScal = kNaFolded + (kNaBlocks / kFold + 1 + 31) / 32 * 32;  // offset of the scalars
+constexpr int64_t kNaDoubles = kNaScal + S_COUNT;
+static_assert(chunk_blocks(kNoAllocRows) == kNaBlocks, "one partial per producer block");
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+// The device address of `scratch` on each device (the array's name is only a
+// host-side handle), looked up once per device by noalloc_scratch_init.
+constexpr int kNaDevices = 64;
+static double *g_scratch[kNaDevices] = {};
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNaDevices) return HPCGX_OK;  // the entry point refuses such devices
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return HPCGX_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HPCGX_EHIP;
+  g_scratch[device] = static_cast<double *>(p);
+  return HPCGX_OK;
+}
+
+static double *noalloc_scratch(int device) {
+  if (device < 0 || device >= kNaDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch[device];
+}
+
 }  // namespace hpcgx
 
 using namespace hpcgx;
@@ -461,6 +501,45 @@ int hpcgx_waxpby_dot(hpcgx_ctx *c, int64_t n, double alpha, const double *x, dou
   HPCGX_CHECK_LAUNCH(c);
   if ((rc = finish(c, 1, chunk_blocks(m), F_DOT))) return rc;
   HPCGX_HIP(c, hipMemcpyAsync(c->hbuf, c->scal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  *result = c->hbuf[0];
+  return HPCGX_OK;
+}
+
+// hpcgx_dot with its partial sums in the static scratch instead of the
+// context's buffers: the same kernels in the same order, so the same bits.
+// Allocates nothing; the problem must already be set up (the context ready).
+int hpcgx_dot_noalloc(hpcgx_ctx *c, int64_t n, const double *x, const double *y, double *result) {
+  if (!c) return HPCGX_EINVAL;
+  if (c->levels.empty() || !c->ready)
+    return set_err(c, HPCGX_ESTATE, "dot_noalloc: no problem set up (it allocates nothing, so it cannot set one up)");
+  if (!result) return set_err(c, HPCGX_EINVAL, "dot_noalloc: null argument");
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(c, HPCGX_EHIP, "hipSetDevice failed");
+  const Layout &lay = c->levels[0].lay;
+  if (n != lay.n && n != lay.npad)
+    return set_err(c, HPCGX_EINVAL, "dot_noalloc: n must be the level-0 row count (" +
+                                        std::to_string((long long)lay.n) + ")");
+  const int64_t m = lay.npad;
+  if (m > kNoAllocRows)
+    return set_err(c, HPCGX_EINVAL, "dot_noalloc: the level has " + std::to_string((long long)m) +
+                                        " padded rows; its static scratch covers at most 2^28 (use hpcgx_dot)");
+  double *s = noalloc_scratch(c->device);
+  if (!s) return set_err(c, HPCGX_ESTATE, "dot_noalloc: no static scratch on this device");
+  int rc = check_vecs(c, m, "dot_noalloc", {x, y});
+  if (rc) return rc;
+  int64_t grid = chunk_blocks(m);
+  k_dot<<<(unsigned)grid, kBlock, 0, c->stream>>>(m, x, y, s);
+  HPCGX_CHECK_LAUNCH(c);
+  const double *src = s;
+  if (grid > kFoldAbove) {
+    const int64_t g2 = (grid + kFold - 1) / kFold;
+    k_fold<1><<<(unsigned)g2, kBlock, 0, c->stream>>>(s, grid, s + kNaFolded);
+    src = s + kNaFolded;
+    grid = g2;
+  }
+  k_finish<1><<<1, kFinishThreads, 0, c->stream>>>(src, grid, s + kNaScal, F_DOT, nullptr);
+  HPCGX_CHECK_LAUNCH(c);
+  HPCGX_HIP(c, hipMemcpyAsync(c->hbuf, s + kNaScal + S_DOT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   *result = c->hbuf[0];
   return HPCGX_OK;

@@ -1112,7 +1112,7 @@ int hpcgx_create(int device, hpcgx_ctx **out) {
   }
   c->stream = c->own_stream;
   int rc;
-  if ((rc = ensure_reduce(c, 4096)) ||
+  if ((rc = noalloc_scratch_init(device)) || (rc = ensure_reduce(c, 4096)) ||
       (rc = dalloc(c, (void **)&c->scal, S_COUNT * sizeof(double)))) {
     hpcgx_destroy(c);
     return rc;

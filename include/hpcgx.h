@@ -208,6 +208,16 @@ int hpcgx_waxpby(hpcgx_ctx *ctx, int64_t n, double alpha, const double *x, doubl
  * the pair WAXPBY+DOT of CG.cpp:122-123). */
 int hpcgx_waxpby_dot(hpcgx_ctx *ctx, int64_t n, double alpha, const double *x, double beta,
                      const double *y, double *w, double *result);
+/* hpcgx_dot that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and bits as hpcgx_dot, but the partial sums live in a
+ * static 2.1 MB device array that each device holds until the process ends.
+ * The problem must already be set up (HPCGX_ESTATE otherwise), and level 0
+ * may have at most 2^28 padded rows (HPCGX_EINVAL beyond).  Every call on one
+ * device shares that array: calls that overlap on one device (other threads,
+ * other contexts' streams) race and must be serialised by the caller. */
+int hpcgx_dot_noalloc(hpcgx_ctx *ctx, int64_t n, const double *x, const double *y, double *result);
 
 /* ---- solver (CG, CG.cpp:59-143) ---------------------------------------- */
 
