@@ -55,6 +55,7 @@ REF_REDUCTION = {16: 3.66234e-41, 192: 5.31511e-05}
 # 1 GiB copy), keyed by grid and column format.  These are profile constants
 # quoted with their source, not measured by this run.
 PROFILED_TRAFFIC = {
+    (192, "compact"): (269098738, "profiles/k_gs_fine_192_fp64_pmc_traffic.md"),
     (192, "explicit"): (361617890, "profiles/k_gs_fine_192_fp64_pmc_traffic.md"),
 }
 
@@ -530,7 +531,11 @@ def leg_general(stream, g=128):
     b = hpcg_rhs(g, g, g)
     res = {}
     try:
-        for name, ctx, bb in (("stencil layout", st, b), ("general layout", gen, b[np.argsort(perms[0])])):
+        for name, ctx, bb, fmt in (("stencil layout", st, b, hpcg_amd.HPCGX_COLS_COMPACT),
+                                   ("general layout", gen, b[np.argsort(perms[0])], hpcg_amd.HPCGX_COLS_COMPACT),
+                                   ("general layout, explicit columns", gen, b[np.argsort(perms[0])],
+                                    hpcg_amd.HPCGX_COLS_EXPLICIT)):
+            ctx.set_column_format(fmt)
             db, dx = ctx.vec(0, bb), ctx.vec(0)
             el, o = run_sets(ctx, db, dx, 50, 3, 1)
             el /= 3
@@ -538,7 +543,8 @@ def leg_general(stream, g=128):
             cols_ = [ctx.colouring(k)[1] for k in range(4)]
             res[name] = {"ms_per_set": round(el * 1e3, 2), "gflops": round(gfl, 1),
                          "scaled_residual_50": float(f"{o['normr'] / o['normr0']:.4e}"), "colours": cols_,
-                         "column_forms_level0": ctx.column_forms(0)}
+                         "column_forms_level0": ctx.column_forms(0),
+                         "columns_read": [ctx.column_reads(k) for k in range(4)]}
             log(f"[bench general] {name}: {el * 1e3:.1f} ms/set, {gfl:.1f} GFLOP/s, colours {cols_}, "
                 f"residual {o['normr'] / o['normr0']:.3e}")
     finally:

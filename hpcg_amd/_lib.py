@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhpcgx.so
 
 HPCGX_PREC_FP64, HPCGX_PREC_FP32, HPCGX_PREC_FP16 = 0, 1, 2
 HPCGX_CG_FP64, HPCGX_CG_MIXED_FP32, HPCGX_CG_MIXED_FP16 = 0, 1, 2
-HPCGX_COLS_EXPLICIT, HPCGX_COLS_COMPACT = 0, 1
+HPCGX_COLS_EXPLICIT, HPCGX_COLS_COMPACT, HPCGX_COLS_COMPACT_ALWAYS = 0, 1, 2
 
 ERRORS = {0: "ok", -1: "EINVAL", -2: "ENOTSTENCIL", -3: "EHIP", -4: "ENOMEM", -5: "ESTATE"}
 
@@ -245,14 +245,21 @@ class Context:
         self._call("hpcgx_prepare_precision", self.h, precision)
 
     def set_column_format(self, fmt):
-        """HPCGX_COLS_COMPACT (default) or HPCGX_COLS_EXPLICIT."""
+        """HPCGX_COLS_COMPACT (default), HPCGX_COLS_COMPACT_ALWAYS or HPCGX_COLS_EXPLICIT."""
         self._call("hpcgx_set_column_format", self.h, fmt)
 
     def column_forms(self, level=0):
-        """Slices of a level per column form: {'delta', 'explicit', 'uniform'}."""
-        out = np.zeros(3, np.int64)
+        """Slices of a level per column form: {'delta', 'explicit', 'uniform'}
+        (see column_reads for the form the kernels read)."""
+        out = np.zeros(4, np.int64)
         self._call("hpcgx_level_column_forms", self.h, level, out)
         return {"delta": int(out[0]), "explicit": int(out[1]), "uniform": int(out[2])}
+
+    def column_reads(self, level=0):
+        """'compact' if the level's kernels read the compact copy, else 'explicit'."""
+        out = np.zeros(4, np.int64)
+        self._call("hpcgx_level_column_forms", self.h, level, out)
+        return "compact" if out[3] else "explicit"
 
     def memory_bytes(self):
         return lib().hpcgx_memory_bytes(self.h)

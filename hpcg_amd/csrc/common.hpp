@@ -33,7 +33,7 @@ enum ColForm : uint8_t {
 };
 
 // Column index format of a context (hpcgx_set_column_format).
-enum ColFormat : int { kColsExplicit = 0, kColsCompact = 1 };
+enum ColFormat : int { kColsExplicit = 0, kColsCompact = 1, kColsCompactAlways = 2 };
 
 // Device scalars of the CG recurrences (Ctx::scal).
 enum Scalar : int {
@@ -157,6 +157,10 @@ struct Level {
   int *cbase = nullptr;
   int8_t *cdelta = nullptr;
   uint8_t *cform = nullptr;
+  // the kernels read the compact copy on this level: fewer than half of its
+  // slices are wide (on a level of mostly wide slices the explicit indices
+  // ran faster: a renumbered 128^3 hierarchy, 779 against 747 GFLOP/s)
+  bool read_compact = false;
   float *dinv = nullptr;   // fp16 copy: 1/sqrt(a_ii), colour order
   float *sigma = nullptr;  // fp16 copy: sqrt(d_f/d_c) per injected row
   int *perm = nullptr;

@@ -50,17 +50,26 @@ __device__ __forceinline__ int64_t tile_id() {
 }
 
 // Occupancy of the fp64 sparse kernels (SpMV, Gauss-Seidel steps,
-// restriction): at most two waves per SIMD, so a wave holds the column,
-// value and gather loads of all its slots in flight (169 VGPRs) instead of
-// the default's batches of about 9 at six waves per SIMD.  Measured on the
-// HPCG sets with explicit indices: 3 % faster at 512^3 (the CG SpMV 9 %),
-// 1.7 % at 192^3; eight waves per SIMD (64 VGPRs) is 10 % slower.  The fp32 /
-// fp16 kernels keep the default (slower with the limit), and so do steps
-// that read fewer than 8 slots (the backend pads every kernel under the
-// limit to its register budget).
+// restriction).  With explicit 32-bit column indices: at most two waves per
+// SIMD, so a wave holds the column, value and gather loads of all its slots
+// in flight (169 VGPRs) instead of the default's batches of about 9 at six
+// waves per SIMD (measured: 3 % faster at 512^3, the CG SpMV 9 %, 1.7 % at
+// 192^3; eight waves per SIMD was 10 % slower).  With compact columns
+// (the default) a uniform slice needs no index load, all 54 loads of a row
+// still issue before the first wait at 122 VGPRs, and no limit is best:
+// measured in alternating processes, the 192^3 sets ran 1245-1249 GFLOP/s
+// unlimited, 1248-1249 at four waves per SIMD, 1241 at three and 1181 at
+// two; the 512^3 FP64 solve 37.9-38.3 ms per iteration unlimited against
+// 40.7-41.5 at two.  HPCGX_CI_FP64_WAVES overrides it at build time.  The
+// fp32 / fp16 kernels keep the default, and so do steps that read fewer
+// than 8 slots (the backend pads every kernel under the limit to its
+// register budget).
+#ifndef HPCGX_CI_FP64_WAVES
+#define HPCGX_CI_FP64_WAVES 8
+#endif
 template <class VT, bool CI = false, int SLOTS = 27>
 struct SparseOccupancy {
-  static constexpr unsigned max_waves = (sizeof(VT) == 8 && SLOTS >= 8) ? 2 : 8;
+  static constexpr unsigned max_waves = (sizeof(VT) == 8 && SLOTS >= 8) ? (CI ? HPCGX_CI_FP64_WAVES : 2) : 8;
 };
 #define HPCGX_SPARSE_OCCUPANCY(VT, CI) __attribute__((amdgpu_waves_per_eu(1, SparseOccupancy<VT, CI>::max_waves)))
 
@@ -102,9 +111,11 @@ struct ColIdx {
 };
 
 // The columns a launch reads on a level: the compact form when the context
-// uses it and the level has it, else the explicit indices.
+// uses it and the level reads it (Level::read_compact), else the explicit
+// indices.
 inline ColIdx col_idx(const Ctx *c, const Level &L) {
-  if (c->col_format == kColsCompact && L.cbase) return ColIdx{L.cols, L.cbase, L.cdelta, L.cform};
+  if (L.cbase && ((c->col_format == kColsCompact && L.read_compact) || c->col_format == kColsCompactAlways))
+    return ColIdx{L.cols, L.cbase, L.cdelta, L.cform};
   return ColIdx{L.cols, nullptr, nullptr, nullptr};
 }
 

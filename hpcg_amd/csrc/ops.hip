@@ -26,10 +26,12 @@ static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, in
   const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
   if (nr <= 0) return;
   const ColIdx ci = col_idx(c, L);
-  // Coarse fp64 steps run at two waves per SIMD (see HPCGX_SPARSE_OCCUPANCY);
-  // launched as one-wave blocks, those of blocks up to 2^20 rows spread over
-  // more CUs: a 192^3 set measured 2-6 % faster, 512^3 about 5 %.  The fine
-  // level, the other precisions and the r.z form keep 256-thread blocks.
+  // Coarse fp64 steps of blocks up to 2^20 rows launch as one-wave blocks,
+  // which spreads them over more CUs: with explicit columns at two waves per
+  // SIMD a 192^3 set measured 2-6 % faster, 512^3 about 5 %; with compact
+  // columns and no occupancy limit, 1245 against 1219 GFLOP/s at 192^3 (512^3
+  // within its run-to-run spread).  The fine level, the other precisions and
+  // the r.z form keep 256-thread blocks.
   const unsigned bs = (!FINE && sizeof(VT) == 8 && nr < ((int64_t)1 << 20)) ? kSlice : kBlock;
   const unsigned g = (unsigned)((nr + bs - 1) / bs);
   hipStream_t st = c->stream;

@@ -1010,12 +1010,18 @@ static int build_compact_cols(Ctx *c, Level &L) {
   k_compact_cols<<<grid, kBlock, 0, c->stream>>>(L.cols, L.vals, L.dslot, L.cbase, L.cdelta, L.cform, ns,
                                                  L.lay.npad);
   HPCGX_CHECK_LAUNCH(c);
+  std::vector<uint8_t> f(ns);
+  HPCGX_HIP(c, hipMemcpyAsync(f.data(), L.cform, ns, hipMemcpyDeviceToHost, c->stream));
+  HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+  int64_t wide = 0;
+  for (const uint8_t v : f) wide += v == kFormWide;
+  L.read_compact = 2 * wide < ns;
   return HPCGX_OK;
 }
 
 // The compact index copies of every level (the default column format).
 static int build_compact_all(Ctx *c) {
-  if (c->col_format != kColsCompact) return HPCGX_OK;
+  if (c->col_format == kColsExplicit) return HPCGX_OK;
   for (auto &L : c->levels) {
     int rc = build_compact_cols(c, L);
     if (rc) return rc;
@@ -1369,10 +1375,12 @@ int hpcgx_prepare_precision(hpcgx_ctx *c, int precision) {
 
 int hpcgx_set_column_format(hpcgx_ctx *c, int format) {
   if (!c) return HPCGX_EINVAL;
-  if (format != HPCGX_COLS_EXPLICIT && format != HPCGX_COLS_COMPACT)
-    return set_err(c, HPCGX_EINVAL, "set_column_format: HPCGX_COLS_EXPLICIT or HPCGX_COLS_COMPACT");
+  if (format != HPCGX_COLS_EXPLICIT && format != HPCGX_COLS_COMPACT && format != HPCGX_COLS_COMPACT_ALWAYS)
+    return set_err(c, HPCGX_EINVAL,
+                   "set_column_format: HPCGX_COLS_EXPLICIT, HPCGX_COLS_COMPACT or HPCGX_COLS_COMPACT_ALWAYS");
   HPCGX_HIP(c, hipSetDevice(c->device));
-  c->col_format = format == HPCGX_COLS_COMPACT ? kColsCompact : kColsExplicit;
+  c->col_format = format == HPCGX_COLS_COMPACT ? kColsCompact
+                 : format == HPCGX_COLS_COMPACT_ALWAYS ? kColsCompactAlways : kColsExplicit;
   if (c->ready) {
     int rc = build_compact_all(c);
     if (rc) return rc;
@@ -1386,6 +1394,7 @@ int hpcgx_level_column_forms(const hpcgx_ctx *c, int level, int64_t *counts) {
   if (level < 0 || level >= (int)c->levels.size()) return HPCGX_EINVAL;
   const Level &L = c->levels[level];
   counts[0] = counts[1] = counts[2] = 0;
+  counts[3] = L.cbase && ((c->col_format == kColsCompact && L.read_compact) || c->col_format == kColsCompactAlways);
   if (!L.cbase) return HPCGX_OK;
   const int64_t ns = L.lay.npad / kSlice;
   std::vector<uint8_t> f(ns);

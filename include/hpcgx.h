@@ -46,6 +46,7 @@ extern "C" {
 /* column index formats (hpcgx_set_column_format) */
 #define HPCGX_COLS_EXPLICIT 0 /* 32-bit column per stored entry, the reference's mtxIndL */
 #define HPCGX_COLS_COMPACT 1  /* per 64-row slice: base + lane, + an int8 delta where needed */
+#define HPCGX_COLS_COMPACT_ALWAYS 2 /* the compact form on every level, even where most slices are wide */
 
 /* hpcgx_cg variants */
 #define HPCGX_CG_FP64 0       /* CG.cpp recurrence, fp64 MG */
@@ -141,13 +142,18 @@ int hpcgx_level_export_packed(hpcgx_ctx *ctx, int level, uint8_t *nnz_row, int *
  * + lane reads no per-entry index, other slices read an int8 delta per entry,
  * and slices whose deltas do not fit in int8 keep the explicit 32-bit
  * indices.  Zero-valued entries may point anywhere in the level.  The form is
- * chosen from the uploaded columns alone.  HPCGX_COLS_EXPLICIT: the 32-bit
- * index of every entry (the reference's mtxIndL).  Results are the same bit
- * for bit; only the bytes read differ. */
+ * chosen from the uploaded columns alone.  A level on which at least half
+ * the slices keep explicit indices reads the explicit indices throughout
+ * (faster there); HPCGX_COLS_COMPACT_ALWAYS reads the compact form on those
+ * levels too.  HPCGX_COLS_EXPLICIT: the 32-bit index of every entry (the
+ * reference's mtxIndL).  Results are the same bit for bit; only the bytes
+ * read differ. */
 int hpcgx_set_column_format(hpcgx_ctx *ctx, int format);
 /* Slices of a level per column form: counts[0] int8 deltas, counts[1]
  * explicit indices, counts[2] no per-entry index (zeros before the compact
- * copy exists). */
+ * copy exists); counts[3] 1 if the level's kernels read the compact copy
+ * (the compact format is on and fewer than half the slices need explicit
+ * indices), else 0.  counts has 4 entries. */
 int hpcgx_level_column_forms(const hpcgx_ctx *ctx, int level, int64_t *counts);
 /* Builds the fp32 or scaled-fp16 copies of every level for the mixed solvers. */
 int hpcgx_prepare_precision(hpcgx_ctx *ctx, int precision);

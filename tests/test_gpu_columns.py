@@ -1,4 +1,4 @@
-"""The two column index formats give the same bits.
+"""The column index formats give the same bits.
 
 The compact format (the default) stores, per 64-row slice and slot, one
 32-bit base, plus an int8 delta per entry where the slice's entries do not
@@ -6,10 +6,12 @@ all sit at base + lane, or keeps the explicit 32-bit indices where a delta
 does not fit in int8.  Zero-valued entries (a neighbour outside the grid)
 may point at any row, their product being an exact zero.  The explicit
 format reads the 32-bit index of every entry, as the reference's mtxIndL.
-Every kernel must give the same bits in both, on stencil levels (all three
-slice forms occur), renumbered and random general levels (delta and wide
-slices) and the variable-coefficient operator, and both are held to the
-oracle elsewhere."""
+A level on which at least half the slices keep explicit indices reads the
+explicit indices under the default; HPCGX_COLS_COMPACT_ALWAYS sends those
+levels through the compact kernels too.  Every kernel must give the same
+bits in all three settings, on stencil levels (all three slice forms occur),
+renumbered and random general levels (delta and wide slices) and the
+variable-coefficient operator, and all are held to the oracle elsewhere."""
 import numpy as np
 import pytest
 
@@ -79,10 +81,16 @@ def test_compact_and_explicit_columns_agree(hip, case):
     try:
         a = _run_all(hip, ctx, hip.HPCGX_COLS_EXPLICIT)
         b = _run_all(hip, ctx, hip.HPCGX_COLS_COMPACT)
+        c = _run_all(hip, ctx, hip.HPCGX_COLS_COMPACT_ALWAYS)   # wide slices through the compact kernels
         for k in a:
             assert np.array_equal(a[k], b[k]), (case, k)
+            assert np.array_equal(a[k], c[k]), (case, k)
         forms = ctx.column_forms(0)
         assert sum(forms.values()) == ctx.info(0)["npad"] // 64
+        # the default reads the compact copy unless half the slices are wide
+        ctx.set_column_format(hip.HPCGX_COLS_COMPACT)
+        want = "compact" if 2 * forms["explicit"] < sum(forms.values()) else "explicit"
+        assert ctx.column_reads(0) == want, (forms, ctx.column_reads(0))
         if case.startswith("hpcg") or case.startswith("kappa"):
             # stencil levels: no slice needs explicit indices, and most need
             # no per-entry index.  The others are the slices whose base + lane
