@@ -23,17 +23,20 @@ The same JSON line also carries:
   * the general layout (a randomly renumbered 128^3 hierarchy, which the
     reference's rule for general matrices, GenerateProblem.cpp:47-49, asks
     to work) against the stencil layout of the same matrices;
+  * the 384^3 variable-coefficient operator: FP16 / FP32 / FP64 MG to 1e-9
+    (FP16 also to 1e-12), forward error against an FP64 solve at 1e-12;
+  * the 128^3 adjoint through torch.autograd: backward / forward time,
+    finite differences on 64 kappa entries, the FP64 gradient;
   * the reference CPU build timed on this host on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--grid N] [--gpus 1]
-                    [--no-512] [--no-general] [--no-tol] [--no-cpu-baseline]
+                    [--no-512] [--no-general] [--no-kappa] [--no-adjoint]
+                    [--no-tol] [--no-cpu-baseline]
 
 Other workloads print their own JSON line:
     --case mixed     512^3 leg alone
-    --case kappa     384^3 variable-coefficient operator, FP16 MG: forward error
-                     vs an FP64 solve at 1e-12
-    --case adjoint   128^3 adjoint through torch.autograd: backward/forward time,
-                     finite differences on 64 kappa entries, fp64 CPU gradient
+    --case kappa     384^3 leg alone
+    --case adjoint   128^3 adjoint leg alone, plus the oracle's FP64 CPU gradient
     --case general   the general-layout leg alone
 """
 import argparse
@@ -310,6 +313,8 @@ def main():
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-1e-9 solves")
     ap.add_argument("--no-512", action="store_true", help="skip the 512^3 leg")
     ap.add_argument("--no-general", action="store_true", help="skip the general-layout leg")
+    ap.add_argument("--no-kappa", action="store_true", help="skip the 384^3 variable-coefficient leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint leg")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--case", default="hpcg", choices=("hpcg", "mixed", "kappa", "adjoint", "general"))
     ap.add_argument("--pmc-calibration", action="store_true",
@@ -439,6 +444,22 @@ def main():
             log(f"[bench] general leg failed: {e!r}")
             general = {"error": repr(e)}
 
+    kap = None
+    if not args.no_kappa:
+        try:
+            kap = leg_kappa(stream, 384)
+        except Exception as e:
+            log(f"[bench] kappa leg failed: {e!r}")
+            kap = {"error": repr(e)}
+
+    adj = None
+    if not args.no_adjoint:
+        try:
+            adj = leg_adjoint(128)
+        except Exception as e:
+            log(f"[bench] adjoint leg failed: {e!r}")
+            adj = {"error": repr(e)}
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -468,6 +489,8 @@ def main():
         "time_to_rel_res_1e-9": tol_res,
         "grid_512": big,
         "general_layout_128": general,
+        "kappa_384": kap,
+        "adjoint_128": adj,
     }
     print(json.dumps(out), flush=True)
 
@@ -575,61 +598,77 @@ def kappa_field(g, seed=2026):
     return kappa, b
 
 
-def case_kappa_fp16(args):
-    """384^3 variable-coefficient operator (log kappa ~ N(0, 0.5^2),
-    b ~ U(-1,1), seeded); MG values stored as scaled FP16 with FP32
-    accumulation, FP64 outer CG.  Forward error against an FP64 solve at 1e-12."""
+def leg_kappa(stream, g=384):
+    """The variable-coefficient operator at g^3 (log kappa ~ N(0, 0.5^2),
+    b ~ U(-1,1), seeded): FP64 MG-PCG to 1e-12 is the reference solve; the
+    FP64-outer flexible PCG with the scaled-FP16 MG (values D^-1/2 A D^-1/2
+    in FP16, FP32 arithmetic) and with the FP32 MG, and FP64 MG-PCG, each to
+    1e-9 (FP16 also to 1e-12): time, iterations, true residual in FP64 and
+    forward error against the reference solve."""
     import numpy as np
-    import torch
 
     import hpcg_amd
-    g = args.grid if args.grid != 192 else 384
     kappa, b = kappa_field(g)
-    stream = torch.cuda.Stream()
     ctx = hpcg_amd.Context(0)
     ctx.set_stream(stream.cuda_stream)
-    t = time.perf_counter()
-    ctx.generate(g, g, g, levels=4, kappa=kappa)
-    ctx.prepare(hpcg_amd.HPCGX_PREC_FP16)
-    ctx.prepare(hpcg_amd.HPCGX_PREC_FP32)
-    ctx.sync()
-    setup = time.perf_counter() - t
-    bn = float(np.linalg.norm(b))
-    db, dx, dref = ctx.vec(0, b), ctx.vec(0), ctx.vec(0)
-    ref = solve_timed(ctx, db, dref, hpcg_amd.HPCGX_CG_FP64, 1e-12)
-    xref = ctx.download(dref)
-    rows = {"fp64_1e-12": {"seconds": round(ref["seconds"], 3), "iterations": ref["niters"],
-                           "true_rel_residual": float(f"{true_relres(ctx, db, dref, bn):.4e}")}}
-    log(f"[bench kappa] fp64 reference: {ref['niters']} it {ref['seconds']:.2f} s")
-    for name, variant in (("mixed_fp16", hpcg_amd.HPCGX_CG_MIXED_FP16), ("mixed_fp32", hpcg_amd.HPCGX_CG_MIXED_FP32),
-                          ("fp64", hpcg_amd.HPCGX_CG_FP64)):
-        for tol in (1e-9, 1e-12):
+    try:
+        t = time.perf_counter()
+        ctx.generate(g, g, g, levels=4, kappa=kappa)
+        ctx.prepare(hpcg_amd.HPCGX_PREC_FP16)
+        ctx.prepare(hpcg_amd.HPCGX_PREC_FP32)
+        ctx.sync()
+        setup = time.perf_counter() - t
+        del kappa
+        bn = float(np.linalg.norm(b))
+        db, dx, dref = ctx.vec(0, b), ctx.vec(0), ctx.vec(0)
+        ref = solve_timed(ctx, db, dref, hpcg_amd.HPCGX_CG_FP64, 1e-12)
+        xref = ctx.download(dref)
+        xn = float(np.linalg.norm(xref))
+        rows = {"fp64_1e-12": {"seconds": round(ref["seconds"], 3), "iterations": ref["niters"],
+                               "true_rel_residual": float(f"{true_relres(ctx, db, dref, bn):.4e}")}}
+        log(f"[bench kappa {g}^3] fp64 reference: {ref['niters']} it {ref['seconds']:.2f} s")
+        for name, variant, tol in (("mixed_fp16", hpcg_amd.HPCGX_CG_MIXED_FP16, 1e-9),
+                                   ("mixed_fp16", hpcg_amd.HPCGX_CG_MIXED_FP16, 1e-12),
+                                   ("mixed_fp32", hpcg_amd.HPCGX_CG_MIXED_FP32, 1e-9),
+                                   ("fp64", hpcg_amd.HPCGX_CG_FP64, 1e-9)):
             o = solve_timed(ctx, db, dx, variant, tol)
-            x = ctx.download(dx)
-            fe = float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
+            fe = float(np.linalg.norm(ctx.download(dx) - xref) / xn)
             rel = true_relres(ctx, db, dx, bn)
             rows[f"{name}_{tol:g}"] = {"seconds": round(o["seconds"], 3), "iterations": o["niters"],
                                        "true_rel_residual": float(f"{rel:.4e}"), "forward_error": float(f"{fe:.4e}")}
-            log(f"[bench kappa] {name} tol {tol:g}: {o['niters']} it {o['seconds']:.2f} s rel {rel:.2e} fwd err {fe:.2e}")
+            log(f"[bench kappa {g}^3] {name} tol {tol:g}: {o['niters']} it {o['seconds']:.2f} s rel {rel:.2e} "
+                f"fwd err {fe:.2e}")
+    finally:
+        ctx.close()
+    return {"grid": [g, g, g], "setup_s": round(setup, 2), "solves": rows,
+            "speedup_fp16_vs_fp64_1e-9": round(rows["fp64_1e-09"]["seconds"] / rows["mixed_fp16_1e-09"]["seconds"], 3),
+            "speedup_fp32_vs_fp64_1e-9": round(rows["fp64_1e-09"]["seconds"] / rows["mixed_fp32_1e-09"]["seconds"], 3)}
+
+
+def case_kappa_fp16(args):
+    import torch
+    g = args.grid if args.grid != 192 else 384
+    res = leg_kappa(torch.cuda.Stream(), g)
     out = {"metric": METRIC, "config": {"workload": f"variable-coefficient 27-point operator {g}^3, "
                                                     f"log kappa ~ N(0, 0.25), b ~ U(-1,1); MG in scaled FP16 values, "
                                                     f"FP32 arithmetic, FP64 outer PCG", "grid": [g, g, g]},
-           "value": rows["mixed_fp16_1e-09"]["forward_error"], "unit": "forward error at rel-res 1e-9 (FP16 MG)",
-           "higher_is_better": False, "solves": rows, "setup_s": round(setup, 2),
-           "speedup_fp16_vs_fp64_1e-9": round(rows["fp64_1e-09"]["seconds"] / rows["mixed_fp16_1e-09"]["seconds"], 3)}
+           "value": res["solves"]["mixed_fp16_1e-09"]["forward_error"],
+           "unit": "forward error at rel-res 1e-9 (FP16 MG)", "higher_is_better": False, **res}
     print(json.dumps(out), flush=True)
 
 
-def case_adjoint(args):
-    """adjoint through the solve at 128^3 (kappa, b drawn as for the 384^3 case);
-    L = 1/2 |x - xt|^2 with xt a seeded N(0,1) field smoothed by 3 SYMGS
-    sweeps.  Forward and adjoint solves: mixed FP32-MG / FP64-outer to 1e-10."""
+def leg_adjoint(g=128, cpu_check=False):
+    """Adjoint through the solve at g^3 (kappa, b drawn as for the kappa
+    leg); L = 1/2 |x - xt|^2 with xt a seeded N(0,1) field smoothed by 3
+    SYMGS sweeps.  Forward and adjoint solves: mixed FP32-MG / FP64-outer to
+    1e-10.  Checks dL/dkappa against central finite differences on 64 kappa
+    entries (h = 1e-6, FP64 solves to 1e-13) and against the FP64 gradient
+    (FP64 MG-PCG to 1e-13 forward and adjoint, on the device; with
+    cpu_check also the oracle's CPU gradient)."""
     import numpy as np
     import torch
 
-    import hpcg_amd
     from hpcg_amd.adjoint import KappaOperator, KappaSolve, kappa_solve
-    g = args.grid if args.grid != 192 else 128
     kappa, b = kappa_field(g)
     rng = np.random.default_rng(77)
     # target: seeded N(0,1) field, 3 symmetric Gauss-Seidel sweeps with r = 0 on the operator
@@ -641,22 +680,30 @@ def case_adjoint(args):
         op.ctx.symgs(0, zr, xd)
     op.ctx.to_natural(xd, xt, level=0)
     op.ctx.sync()
-    for rep in range(2):  # the first pass warms torch's kernels up; the second is timed
+    op.close()
+
+    def grad(variant, tol):
         k = torch.tensor(kappa, device="cuda", requires_grad=True)
         bt = torch.tensor(b, device="cuda", requires_grad=True)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        x = kappa_solve(k, bt, (g, g, g), 1e-10, "mixed_fp32")
+        x = kappa_solve(k, bt, (g, g, g), tol, variant)
         loss = 0.5 * ((x - xt) ** 2).sum()
         torch.cuda.synchronize()
         t_fwd = time.perf_counter() - t
         t = time.perf_counter()
         loss.backward()
         torch.cuda.synchronize()
-        t_bwd = time.perf_counter() - t
+        return k.grad.detach(), bt.grad.detach(), t_fwd, time.perf_counter() - t
+
+    for rep in range(2):  # the first pass warms torch's kernels up; the second is timed
+        gk, gb, t_fwd, t_bwd = grad("mixed_fp32", 1e-10)
     tim = dict(KappaSolve.last_timing)
-    gk = k.grad.detach()
-    log(f"[bench adjoint] forward {t_fwd:.3f} s backward {t_bwd:.3f} s timing {tim}")
+    log(f"[bench adjoint {g}^3] forward {t_fwd:.3f} s backward {t_bwd:.3f} s timing {tim}")
+    gk64, gb64, _, _ = grad("fp64", 1e-13)
+    dev_rel = float(torch.linalg.norm(gk - gk64) / torch.linalg.norm(gk64))
+    devb_rel = float(torch.linalg.norm(gb - gb64) / torch.linalg.norm(gb64))
+    log(f"[bench adjoint {g}^3] vs FP64 gradient (1e-13): dL/dkappa rel {dev_rel:.2e}, dL/db rel {devb_rel:.2e}")
     # finite differences on 64 kappa entries, fp64 solves to 1e-13
     bd = torch.tensor(b, device="cuda")
     idx = rng.choice(g ** 3, 64, replace=False)
@@ -674,31 +721,37 @@ def case_adjoint(args):
     fd = np.array(fd)
     ga = gk[torch.tensor(idx, device="cuda")].cpu().numpy()
     fd_rel = float(np.max(np.abs(fd - ga)) / np.max(np.abs(ga)))
-    log(f"[bench adjoint] FD max rel err (vs max |g|) {fd_rel:.2e}")
-    # fp64 CPU reference gradient (the oracle, fp64 CG to 1e-13)
-    cpu_rel = None
-    try:
-        from oracle import pyoracle as po
-        P = po.Problem(g, g, g, kappa=kappa)
-        xtn = xt.cpu().numpy()
-        xo = P.cg(b, max_iter=2000, tol=1e-13, mode=2)["x"]
-        lo = P.cg(xo - xtn, max_iter=2000, tol=1e-13, mode=2)["x"]
-        go = P.kappa_grad(lo, xo)
-        cpu_rel = float(np.linalg.norm(gk.cpu().numpy() - go) / np.linalg.norm(go))
-        b_rel = float(np.linalg.norm(bt.grad.cpu().numpy() - lo) / np.linalg.norm(lo))
-        log(f"[bench adjoint] vs fp64 CPU gradient: dL/dkappa rel {cpu_rel:.2e}, dL/db rel {b_rel:.2e}")
-    except Exception as e:
-        log(f"[bench adjoint] CPU gradient failed: {e}")
-        b_rel = None
+    log(f"[bench adjoint {g}^3] FD max rel err (vs max |g|) {fd_rel:.2e}")
+    out = {"grid": [g, g, g], "backward_over_forward": round(t_bwd / t_fwd, 3),
+           "solve_ratio_adjoint_over_forward": round(tim["adjoint_solve_s"] / tim["forward_solve_s"], 3),
+           "forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4), "timing": tim,
+           "fd_64_entries_max_rel_err": fd_rel, "fp64_gradient_rel_err": dev_rel, "fp64_db_rel_err": devb_rel}
+    if cpu_check:
+        # fp64 CPU reference gradient (the oracle, fp64 CG to 1e-13): the checker
+        try:
+            from oracle import pyoracle as po
+            P = po.Problem(g, g, g, kappa=kappa)
+            xtn = xt.cpu().numpy()
+            xo = P.cg(b, max_iter=2000, tol=1e-13, mode=2)["x"]
+            lo = P.cg(xo - xtn, max_iter=2000, tol=1e-13, mode=2)["x"]
+            go = P.kappa_grad(lo, xo)
+            out["fp64_cpu_gradient_rel_err"] = float(np.linalg.norm(gk.cpu().numpy() - go) / np.linalg.norm(go))
+            out["fp64_cpu_db_rel_err"] = float(np.linalg.norm(gb.cpu().numpy() - lo) / np.linalg.norm(lo))
+            log(f"[bench adjoint {g}^3] vs fp64 CPU gradient: dL/dkappa rel {out['fp64_cpu_gradient_rel_err']:.2e}, "
+                f"dL/db rel {out['fp64_cpu_db_rel_err']:.2e}")
+        except Exception as e:
+            log(f"[bench adjoint {g}^3] CPU gradient failed: {e}")
+    return out
+
+
+def case_adjoint(args):
+    g = args.grid if args.grid != 192 else 128
+    res = leg_adjoint(g, cpu_check=True)
     out = {"metric": METRIC, "config": {"workload": f"adjoint of the {g}^3 variable-coefficient solve "
                                                     f"through torch.autograd.Function, mixed FP32-MG / FP64-outer "
                                                     f"PCG to 1e-10", "grid": [g, g, g]},
-           "value": round(t_bwd / t_fwd, 3), "unit": "backward / forward time (autograd calls, incl. setup)",
-           "higher_is_better": False,
-           "solve_ratio_adjoint_over_forward": round(tim["adjoint_solve_s"] / tim["forward_solve_s"], 3),
-           "forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4), "timing": tim,
-           "fd_64_entries_max_rel_err": fd_rel, "fp64_cpu_gradient_rel_err": cpu_rel,
-           "fp64_cpu_db_rel_err": b_rel}
+           "value": res["backward_over_forward"], "unit": "backward / forward time (autograd calls, incl. setup)",
+           "higher_is_better": False, **res}
     print(json.dumps(out), flush=True)
 
 

@@ -46,12 +46,22 @@ bool device_of_length(local_int_t n, hpcgx_ctx **ctx, int *level) {
 }  // namespace
 
 // OptimizeProblem.cpp:37: colour, reorder and pack every level on the device.
+// With HPCG_DROPIN_TIMING set, the time of each phase goes to stderr.
 int OptimizeProblem(SparseMatrix &A, CGData &data, Vector &b, Vector &x, Vector &xexact) {
   (void)data;
+  const bool timing = std::getenv("HPCG_DROPIN_TIMING") != nullptr;
+  double t = now_s();
+  auto phase = [&](const char *what) {
+    if (!timing) return;
+    const double t1 = now_s();
+    std::fprintf(stderr, "hpcg dropin: OptimizeProblem %s %.4f s\n", what, t1 - t);
+    t = t1;
+  };
   const char *dev = std::getenv("HPCG_DEVICE");
   hpcgx_ctx *ctx = nullptr;
   int rc = hpcgx_create(dev ? std::atoi(dev) : 0, &ctx);
   if (rc) return fail(nullptr, "OptimizeProblem", rc);
+  phase("context");
   int level = 0;
   for (SparseMatrix *m = &A; m; m = m->Ac, ++level) {
     const local_int_t *f2c = m->mgData ? m->mgData->f2cOperator : nullptr;
@@ -59,13 +69,16 @@ int OptimizeProblem(SparseMatrix &A, CGData &data, Vector &b, Vector &x, Vector 
                             m->mtxIndL, m->matrixValues, f2c);
     if (rc) return fail(ctx, "OptimizeProblem (level upload)", rc);
     m->optimizationData = new hpcgx_dropin_matrix{ctx, level, level == 0};
+    phase(level == 0 ? "level 0 upload" : "coarse level upload");
   }
   if ((rc = hpcgx_prepare_precision(ctx, HPCGX_PREC_FP64))) return fail(ctx, "OptimizeProblem", rc);
+  phase("solver buffers and column forms");
   dropin_set_context(ctx);
   double *d;
   if ((rc = dropin_sync_in(ctx, 0, b, &d)) || (rc = dropin_sync_in(ctx, 0, x, &d)) ||
       (rc = dropin_sync_in(ctx, 0, xexact, &d)))
     return fail(ctx, "OptimizeProblem (vectors)", rc);
+  phase("vectors");
   return 0;
 }
 

@@ -11,8 +11,13 @@
 // * host threads fill disjoint ranges of rows (first touch, so each thread
 //   faults in what it writes), with no lock per row: each thread counts its
 //   own nonzeros;
-// * the global-to-local map is built by one more thread while the others fill
-//   the arrays (an unordered map cannot take concurrent inserts);
+// * the global-to-local map is built only where something reads it: the
+//   reference's MPI halo setup (SetupHalo_ref.cpp:137,150) and its
+//   HPCG_DETAILED_DEBUG prints (CheckProblem.cpp:89, GenerateProblem_ref.cpp:147,
+//   SetupHalo_ref.cpp:86).  A one-rank build without those never reads it,
+//   and its 7M single-threaded inserts at 192^3 (0.3 s; an unordered map
+//   cannot take concurrent inserts) were most of the setup time.  Where it is
+//   built, one more thread builds it while the others fill the arrays;
 // * SetupHalo's one-rank index copy (SetupHalo_ref.cpp:57-65) runs on the
 //   same threads.
 // Setup time is charged to the HPCG rating (times[9], ReportResults.cpp:353),
@@ -32,6 +37,12 @@
 #include "dropin_state.hpp"
 
 namespace {
+
+#if !defined(HPCG_NO_MPI) || defined(HPCG_DETAILED_DEBUG)
+constexpr bool kBuildGlobalToLocalMap = true;
+#else
+constexpr bool kBuildGlobalToLocalMap = false;
+#endif
 
 struct Storage {
   void *vals, *indG, *indL;
@@ -124,15 +135,17 @@ void GenerateProblem(SparseMatrix &A, Vector *b, Vector *x, Vector *xexact) {
   A.localToGlobalMap.resize(n);
   global_int_t *l2g = A.localToGlobalMap.data();
 
-  // the global-to-local map on its own thread, in row order
-  std::thread map_thread([&A, n, nx, ny, gnx, gny, gix0, giy0, giz0] {
-    A.globalToLocalMap.clear();
-    A.globalToLocalMap.reserve(n);
-    for (local_int_t i = 0; i < n; ++i) {
-      const global_int_t ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
-      A.globalToLocalMap.emplace((giz0 + iz) * gnx * gny + (giy0 + iy) * gnx + gix0 + ix, i);
-    }
-  });
+  // the global-to-local map on its own thread, in row order, where it is read
+  A.globalToLocalMap.clear();
+  std::thread map_thread;
+  if (kBuildGlobalToLocalMap)
+    map_thread = std::thread([&A, n, nx, ny, gnx, gny, gix0, giy0, giz0] {
+      A.globalToLocalMap.reserve(n);
+      for (local_int_t i = 0; i < n; ++i) {
+        const global_int_t ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
+        A.globalToLocalMap.emplace((giz0 + iz) * gnx * gny + (giy0 + iy) * gnx + gix0 + ix, i);
+      }
+    });
 
   const unsigned nt = setup_threads();
   std::vector<global_int_t> counts(nt, 0);
@@ -175,7 +188,7 @@ void GenerateProblem(SparseMatrix &A, Vector *b, Vector *x, Vector *xexact) {
     }
     counts[t] = cnt;
   });
-  map_thread.join();
+  if (map_thread.joinable()) map_thread.join();
   global_int_t nnz = 0;
   for (global_int_t c : counts) nnz += c;
   assert(nnz > 0);

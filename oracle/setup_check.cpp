@@ -6,8 +6,10 @@
 //   1. on every level of the hierarchy (the fine grid and each coarse grid
 //      GenerateCoarseProblem.cpp:50-53 makes), both build the level and every
 //      array and field must be equal: nonzerosInRow, mtxIndG, mtxIndL,
-//      matrixValues, matrixDiagonal (as offsets into the row), the maps, the
-//      counts and b, x, xexact;
+//      matrixValues, matrixDiagonal (as offsets into the row), the local-to-
+//      global map, the counts and b, x, xexact; the global-to-local map too
+//      where the build reads it (MPI or HPCG_DETAILED_DEBUG; else the drop-in
+//      leaves it empty);
 //   2. the reference's driver sequence (main.cpp:131-158) runs on the
 //      drop-in hooks: GenerateProblem, SetupHalo, GenerateCoarseProblem three
 //      times, then CheckProblem on every level (its asserts are live: this
@@ -78,7 +80,13 @@ static void compare_level(local_int_t nx, local_int_t ny, local_int_t nz, bool v
   CHECK(D.totalNumberOfNonzeros == R.totalNumberOfNonzeros, "total nonzeros");
   CHECK(D.title == R.title, "title");
   CHECK(D.localToGlobalMap == R.localToGlobalMap, "localToGlobalMap");
+#if !defined(HPCG_NO_MPI) || defined(HPCG_DETAILED_DEBUG)
   CHECK(D.globalToLocalMap == R.globalToLocalMap, "globalToLocalMap");
+#else
+  // one rank, no detailed debug: nothing reads the map, and the drop-in
+  // leaves it empty (hpcg_amd/dropin/setup_hooks.cpp)
+  CHECK(D.globalToLocalMap.empty(), "globalToLocalMap left empty");
+#endif
   for (local_int_t i = 0; i < n && fails == 0; ++i) {
     const int k = R.nonzerosInRow[i];
     CHECK(D.nonzerosInRow[i] == k, "nonzerosInRow[%d]", i);
