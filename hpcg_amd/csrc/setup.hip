@@ -391,14 +391,78 @@ static int ensure_upload_staging(Ctx *c) {
   return HPCGX_OK;
 }
 
+// Whether every row is exactly the HPCG matrix's row of the grid
+// (GenerateProblem_ref.cpp:152-180): the in-grid neighbours of the 27-point
+// stencil in ascending column order, 26 on the diagonal, -1 off it.  Checked
+// on host threads, which read the rows at host memory speed; *nnz gets the
+// stored entries.  Stops at the first row that differs.
+template <class RowAcc>
+static bool rows_are_hpcg(RowAcc rows, const Layout &lay, int64_t *nnz) {
+  const int64_t nx = lay.nx, ny = lay.ny, nz = lay.nz, n = lay.n;
+  std::atomic<bool> ok(true);
+  std::atomic<int64_t> total(0);
+  parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+    int64_t sum = 0;
+    for (int64_t i = lo; i < hi && ok.load(std::memory_order_relaxed); ++i) {
+      const int64_t ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
+      const int cnt = rows.nnz(i);
+      const int *rc = rows.cols(i);
+      const double *rv = rows.vals(i);
+      int k = 0;
+      bool row_ok = true;
+      for (int sz = -1; sz <= 1 && row_ok; ++sz) {
+        if (iz + sz < 0 || iz + sz >= nz) continue;
+        for (int sy = -1; sy <= 1 && row_ok; ++sy) {
+          if (iy + sy < 0 || iy + sy >= ny) continue;
+          for (int sx = -1; sx <= 1; ++sx) {
+            if (ix + sx < 0 || ix + sx >= nx) continue;
+            const int64_t col = i + (sz * ny + sy) * nx + sx;
+            if (k >= cnt || rc[k] != col || rv[k] != (col == i ? 26.0 : -1.0)) {
+              row_ok = false;
+              break;
+            }
+            ++k;
+          }
+        }
+      }
+      if (!row_ok || k != cnt) {
+        ok = false;
+        break;
+      }
+      sum += cnt;
+    }
+    total += sum;
+  });
+  *nnz = total;
+  return ok;
+}
+
 // Packs a stencil level on the device.  *stencil is false when some row is
 // not the stencil of the grid (the level's device arrays are then released).
+// Rows that are exactly the HPCG matrix (rows_are_hpcg) are not sent: the
+// device writes the same values from the closed form (k_generate, bit-exact
+// against GenerateProblem_ref), which at 192^3 saves moving 2.3 GB over the
+// bus; any other stencil rows are gathered, copied and checked on the device.
 template <class RowAcc>
 static int pack_stencil_device(Ctx *c, Level &L, RowAcc rows, bool *stencil) {
   const Layout &lay = L.lay;
   const int64_t n = lay.n, npad = lay.npad;
   int rc;
   *stencil = false;
+  int64_t hpcg_nnz = 0;
+  if (rows_are_hpcg(rows, lay, &hpcg_nnz)) {
+    if ((rc = alloc_level(c, L))) return rc;
+    k_generate<<<nblk(npad), kBlock, 0, c->stream>>>(L.ldev, npad, L.cols, L.vals, L.iperm, L.perm, nullptr, true);
+    HPCGX_CHECK_LAUNCH(c);
+    L.perm_h.resize(n);
+    L.iperm_h.resize(npad);
+    HPCGX_HIP(c, hipMemcpyAsync(L.perm_h.data(), L.perm, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HPCGX_HIP(c, hipMemcpyAsync(L.iperm_h.data(), L.iperm, npad * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HPCGX_HIP(c, hipStreamSynchronize(c->stream));
+    L.nnz = hpcg_nnz;
+    *stencil = true;
+    return HPCGX_OK;
+  }
   if ((rc = ensure_upload_staging(c)) || (rc = alloc_level(c, L))) return rc;
   k_generate<<<nblk(npad), kBlock, 0, c->stream>>>(L.ldev, npad, L.cols, L.vals, L.iperm, L.perm, nullptr, false);
   HPCGX_CHECK_LAUNCH(c);
@@ -828,6 +892,26 @@ int ensure_ready(Ctx *c) {
     if ((int64_t)F.f2c_h.size() != nc)
       return set_err(c, HPCGX_ESTATE, "level " + std::to_string(l) + " has no f2c operator for the " +
                                           std::to_string((long long)nc) + " rows of level " + std::to_string(l + 1));
+    // Stencil levels whose f2c is the reference's injection (coarse point
+    // (a, b, c) <- fine point (2a, 2b, 2c), GenerateCoarseProblem.cpp:71-82)
+    // on grids that halve exactly take the generated map, checked in one
+    // pass over f2c instead of inverting it over the fine level.
+    if (!F.lay.general && !Cc.lay.general && F.lay.nx == 2 * Cc.lay.nx && F.lay.ny == 2 * Cc.lay.ny &&
+        F.lay.nz == 2 * Cc.lay.nz) {
+      const int64_t cx = Cc.lay.nx, cy = Cc.lay.ny, fx = F.lay.nx, fy = F.lay.ny;
+      bool closed = true;
+      for (int64_t k = 0; closed && k < nc; ++k) {
+        const int64_t a = k % cx, b = (k / cx) % cy, cz = k / (cx * cy);
+        closed = F.f2c_h[k] == ((2 * cz) * fy + 2 * b) * fx + 2 * a;
+      }
+      if (closed) {
+        F.ninj = nc;
+        if ((rc = dalloc(c, (void **)&F.inj, nc * sizeof(int), false))) return rc;
+        k_inj_generated<<<nblk(nc), kBlock, 0, c->stream>>>(F.ldev, Cc.ldev, F.inj, nc);
+        HPCGX_CHECK_LAUNCH(c);
+        continue;
+      }
+    }
     std::vector<int> inv(F.lay.npad, -1);
     for (int64_t k = 0; k < nc; ++k) {
       const int f = F.f2c_h[k];
@@ -1101,6 +1185,12 @@ using namespace hpcgx;
 // C ABI
 
 extern "C" {
+
+int hpcgx_device_init(int device) {
+  if (hipSetDevice(device) != hipSuccess) return HPCGX_EHIP;
+  if (hipFree(nullptr) != hipSuccess) return HPCGX_EHIP;  // brings up the device's context
+  return HPCGX_OK;
+}
 
 int hpcgx_create(int device, hpcgx_ctx **out) {
   if (!out) return HPCGX_EINVAL;

@@ -32,5 +32,10 @@ int dropin_sync_in(hpcgx_ctx *ctx, int level, const Vector &v, double **dptr);
 // copies the device result to the host; both copies then agree
 int dropin_sync_out(hpcgx_ctx *ctx, int level, Vector &v);
 void dropin_release_all(hpcgx_ctx *ctx);
+// the device the hooks use: $HPCG_DEVICE, default 0
+int dropin_device();
+// waits for the device start-up that GenerateProblem began and returns the
+// context it opened (once; null if it did not start or failed)
+hpcgx_ctx *dropin_take_started_context();
 
 #endif

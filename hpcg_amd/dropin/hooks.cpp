@@ -57,10 +57,10 @@ int OptimizeProblem(SparseMatrix &A, CGData &data, Vector &b, Vector &x, Vector 
     std::fprintf(stderr, "hpcg dropin: OptimizeProblem %s %.4f s\n", what, t1 - t);
     t = t1;
   };
-  const char *dev = std::getenv("HPCG_DEVICE");
-  hpcgx_ctx *ctx = nullptr;
-  int rc = hpcgx_create(dev ? std::atoi(dev) : 0, &ctx);
-  if (rc) return fail(nullptr, "OptimizeProblem", rc);
+  hpcgx_ctx *ctx = dropin_take_started_context();
+  phase("device start-up (rest of)");
+  int rc = 0;
+  if (!ctx && (rc = hpcgx_create(dropin_device(), &ctx))) return fail(nullptr, "OptimizeProblem", rc);
   phase("context");
   int level = 0;
   for (SparseMatrix *m = &A; m; m = m->Ac, ++level) {

@@ -20,6 +20,11 @@
 //   built, one more thread builds it while the others fill the arrays;
 // * SetupHalo's one-rank index copy (SetupHalo_ref.cpp:57-65) runs on the
 //   same threads.
+// * the HIP runtime, the device and the hpcgx context start on one more
+//   thread at the first GenerateProblem call (hpcgx_device_init, then
+//   hpcgx_create), so that start-up overlaps the host fill instead of opening
+//   OptimizeProblem (runtime 38 ms on a warm box, 175 ms on a fresh one,
+//   context 31 ms, measured); OptimizeProblem takes the context.
 // Setup time is charged to the HPCG rating (times[9], ReportResults.cpp:353),
 // which is why it matters here.  DeleteMatrix (hpcg_dropin.hpp) frees this
 // storage through HPCG_MatrixStorageRelease.
@@ -47,6 +52,18 @@ constexpr bool kBuildGlobalToLocalMap = false;
 struct Storage {
   void *vals, *indG, *indL;
 };
+
+// The device start-up thread and the context it opens; joined by
+// OptimizeProblem, or at exit.
+struct DeviceInit {
+  std::mutex mu;
+  std::thread th;
+  bool started = false;
+  hpcgx_ctx *ctx = nullptr;  // opened by the thread, taken by OptimizeProblem
+  ~DeviceInit() {
+    if (th.joinable()) th.join();
+  }
+} g_device_init;
 std::mutex g_store_mu;
 std::unordered_map<const void *, Storage> g_store;  // keyed by the values block
 
@@ -83,6 +100,19 @@ void *big_alloc(size_t bytes) {
 
 }  // namespace
 
+int dropin_device() {
+  const char *dev = std::getenv("HPCG_DEVICE");
+  return dev ? std::atoi(dev) : 0;
+}
+
+hpcgx_ctx *dropin_take_started_context() {
+  std::lock_guard<std::mutex> lk(g_device_init.mu);
+  if (g_device_init.th.joinable()) g_device_init.th.join();
+  hpcgx_ctx *ctx = g_device_init.ctx;
+  g_device_init.ctx = nullptr;
+  return ctx;
+}
+
 extern "C" int HPCG_MatrixStorageRelease(SparseMatrix *A) {
   if (!A || !A->matrixValues || A->localNumberOfRows <= 0) return 0;
   Storage s;
@@ -104,6 +134,17 @@ extern "C" int HPCG_MatrixStorageRelease(SparseMatrix *A) {
 // 26 on the diagonal and -1 off it, entries in ascending column order,
 // b_i = 26 - (nnz_i - 1), x = 0, xexact = 1.
 void GenerateProblem(SparseMatrix &A, Vector *b, Vector *x, Vector *xexact) {
+  {
+    std::lock_guard<std::mutex> lk(g_device_init.mu);
+    if (!g_device_init.started) {
+      g_device_init.started = true;
+      const int dev = dropin_device();
+      g_device_init.th = std::thread([dev] {
+        hpcgx_ctx *ctx = nullptr;
+        if (hpcgx_device_init(dev) == HPCGX_OK && hpcgx_create(dev, &ctx) == HPCGX_OK) g_device_init.ctx = ctx;
+      });
+    }
+  }
   const global_int_t nx = A.geom->nx, ny = A.geom->ny, nz = A.geom->nz;
   const global_int_t gnx = A.geom->gnx, gny = A.geom->gny, gnz = A.geom->gnz;
   const global_int_t gix0 = A.geom->gix0, giy0 = A.geom->giy0, giz0 = A.geom->giz0;

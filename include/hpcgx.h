@@ -61,6 +61,12 @@ typedef struct hpcgx_ctx hpcgx_ctx;
  * reference keeps behind SparseMatrix::optimizationData
  * (SparseMatrix.hpp:58-64, OptimizeProblem.cpp:37). */
 int hpcgx_create(int device, hpcgx_ctx **out);
+/* Starts the HIP runtime and the device's context ahead of hpcgx_create (no
+ * reference counterpart: the reference has no device).  Safe to call from
+ * another thread while the caller does host work; the drop-in's
+ * GenerateProblem hook runs it that way, so the runtime comes up while the
+ * host builds the matrix instead of at the start of OptimizeProblem. */
+int hpcgx_device_init(int device);
 /* Frees every device buffer of the context (the reference's DeleteMatrix,
  * SparseMatrix.hpp:151-183, for the device copies). */
 int hpcgx_destroy(hpcgx_ctx *ctx);

@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "hpcgx.h"
 #include "CheckProblem.hpp"
 #include "GenerateCoarseProblem.hpp"
 #include "GenerateGeometry.hpp"
@@ -38,6 +39,11 @@ void HPCG_VectorHostZeroed(Vector *) {}
 void HPCG_VectorRelease(Vector *) {}
 void HPCG_MatrixDiagonalReplaced(SparseMatrix *, const double *) {}
 void HPCG_MatrixRelease(SparseMatrix *) {}
+int hpcgx_device_init(int) { return 0; }
+int hpcgx_create(int, hpcgx_ctx **out) {
+  *out = nullptr;
+  return -3;  // no device here: the drop-in then opens none
+}
 }
 
 static int fails = 0;

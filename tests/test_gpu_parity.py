@@ -72,6 +72,49 @@ def test_upload_equals_generate(hip):
     assert np.array_equal(ys[0], ys[1])
 
 
+def test_upload_stencil_rows_with_other_values(hip):
+    """Rows with the stencil's pattern but values other than the HPCG
+    matrix's are copied and checked on the device (the HPCG matrix itself is
+    recognised on the host and written from the closed form): the kappa
+    operator uploaded equals the kappa operator generated, and a HPCG level
+    with one diagonal entry changed keeps that entry, in the stencil layout."""
+    kappa = np.exp(0.5 * rng(11).standard_normal(16 ** 3))
+    P = po.Problem(16, 16, 16, kappa=kappa)
+    up = hip.Context(0)
+    for l in range(4):
+        lv = P.level(l)
+        nr, cols, vals, f2c = lv.arrays()
+        up.upload_level(l, lv.dims, nr, cols, vals, f2c)
+    gen = hip.Context(0).generate(16, 16, 16, kappa=kappa)
+    for l in range(4):
+        assert not up.colouring(l)[2]          # stencil layout
+        for u, v in zip(up.export_level(l), gen.export_level(l)):
+            assert np.array_equal(u, v), l
+    r = rng(12).random(P.n)
+    zs = []
+    for ctx in (up, gen):
+        dr, dz = ctx.vec(0, r), ctx.vec(0)
+        ctx.mg(0, dr, dz)
+        zs.append(dz.download())
+    assert np.array_equal(zs[0], zs[1])
+    assert np.array_equal(zs[0], P.mg(r, mode=2))
+    up.close()
+    gen.close()
+    H = po.Problem(8, 8, 8, levels=1)
+    nr, cols, vals, _ = H.arrays()
+    vals = vals.copy()
+    i = 100
+    j = int(np.nonzero(cols[i, :nr[i]] == i)[0][0])
+    vals[i, j] = 27.5
+    ctx = hip.Context(0)
+    ctx.upload_level(0, H.dims, nr, cols, vals, None)
+    assert not ctx.colouring(0)[2]
+    enr, ecols, evals = ctx.export_level(0)
+    mask = np.arange(27)[None, :] < nr[:, None]
+    assert np.array_equal(evals[mask], vals[mask]) and np.array_equal(ecols[mask], cols[mask])
+    ctx.close()
+
+
 def test_upload_non_stencil_takes_general_layout(hip):
     """A row that is not the stencil of the stated grid: the level is
     uploaded in the general layout (not rejected) and matches the oracle's
