@@ -417,7 +417,10 @@ void k_gs(const ColIdx ci, const VT *__restrict__ vals,
   static_assert(!RZ || ZK < 0, "r.z is taken from the post-smoother");
   // blocks of 256 threads, or of 64 for the small coarse fp64 steps (one wave
   // per block spreads them over more CUs); the r.z form always takes 256
-  const int64_t t = tile_id<FINE && sizeof(VT) == 8>() * (int64_t)blockDim.x + threadIdx.x;
+#ifndef HPCGX_FINE_ROT
+#define HPCGX_FINE_ROT 1
+#endif
+  const int64_t t = tile_id<HPCGX_FINE_ROT && FINE && sizeof(VT) == 8>() * (int64_t)blockDim.x + threadIdx.x;
   if constexpr (RZ) {
     // every thread reaches the block reduction; rows past the block add 0
     double acc = 0.0;

@@ -102,7 +102,10 @@ Layout make_layout(int nx, int ny, int nz, const int *order) {
   const int nby = L.nb[0][1];
   bool strips = ny % 2 == 0 && nby >= 128 && nby % 8 == 0;
   for (int k = 1; k < kColours; ++k) strips = strips && L.nb[k][1] == nby;
-  L.strip = strips ? nby / 8 : 0;
+#ifndef HPCGX_STRIPS
+#define HPCGX_STRIPS 1
+#endif
+  L.strip = HPCGX_STRIPS && strips ? nby / 8 : 0;
   return L;
 }
 
