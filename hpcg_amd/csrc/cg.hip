@@ -67,12 +67,12 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(double, CI) void k_c
   if (row < nrows) {
     const size_t b = ell_base(row);
     const int slice = slice_of<true>(row);
-    const double sum = with_cols<CI, true>(ci, slice, [&](auto cm) {
+    const double sum = with_cols<CI, true>(ci, slice, [&](auto cm, const int *base) {
       constexpr int CM = decltype(cm)::value;
       double acc_row = 0.0;
 #pragma unroll
       for (int s = 0; s < kSlots; ++s) {
-        const int c = col_at<CM, true>(ci, row, slice, b, s);
+        const int c = col_at<CM, true>(ci, row, base, b, s);
         const double v = ld_val<double>(vals + b + s * kSlice);
         acc_row += v * p[c];
       }

@@ -32,6 +32,11 @@ enum ColForm : uint8_t {
   kFormUniform = 2,  // every stored entry of each slot sits at base[slot] + lane: no per-entry index
 };
 
+// Ints per slice in the compact index record: the 27 slot bases, then the
+// slice's form (ColForm), then padding to 128 bytes.  A wave reads a slice's
+// form and bases with scalar loads from one record, in one round trip.
+constexpr int kMeta = 32;
+
 // Column index format of a context (hpcgx_set_column_format).
 enum ColFormat : int { kColsExplicit = 0, kColsCompact = 1, kColsCompactAlways = 2 };
 
@@ -150,10 +155,12 @@ struct Level {
   float *vals32 = nullptr;
   __half *vals16 = nullptr;
   // compact column indices (the default column format of every copy): per
-  // slice a form (cform[slice], ColForm) and per slice and slot a 32-bit
-  // base; the column of (row, slot) is cbase[slice*27+slot] + lane (uniform
-  // slices), that plus the int8 cdelta[ell] (delta slices), or cols[ell]
-  // (wide slices, whose deltas do not fit in int8)
+  // slice a record of kMeta ints (cbase: the 32-bit base of each slot, then
+  // the slice's form, ColForm), and the form again per slice in cform (for
+  // launches whose waves span several slices, and the host's counts); the
+  // column of (row, slot) is cbase[slice*kMeta+slot] + lane (uniform slices),
+  // that plus the int8 cdelta[ell] (delta slices), or cols[ell] (wide slices,
+  // whose deltas do not fit in int8)
   int *cbase = nullptr;
   int8_t *cdelta = nullptr;
   uint8_t *cform = nullptr;

@@ -130,38 +130,52 @@ __device__ __forceinline__ int slice_of(int64_t row) {
 
 // Column of (row, slot s).  CM 0: the explicit 32-bit index; CM 1: base +
 // lane + int8 delta; CM 2: base + lane.  A launch picks one per slice
-// (with_cols), so each inner loop is straight-line code.
+// (with_cols), so each inner loop is straight-line code; base holds the
+// slice's 27 slot bases (with_cols provides them).
 template <int CM, bool NT>
-__device__ __forceinline__ int col_at(const ColIdx &ci, int64_t row, int slice, size_t b, int s) {
+__device__ __forceinline__ int col_at(const ColIdx &ci, int64_t row, const int *base, size_t b, int s) {
   if constexpr (CM == 0) {
     return ld_col<NT>(ci.cols + b + s * kSlice);
   } else if constexpr (CM == 1) {
-    const int base = ci.cbase[(int64_t)slice * kSlots + s];
     int8_t d;
     if constexpr (NT) d = __builtin_nontemporal_load(ci.cdelta + b + s * kSlice);
     else d = ci.cdelta[b + s * kSlice];
-    return base + (int)(row & 63) + (int)d;
+    return base[s] + (int)(row & 63) + (int)d;
   } else {
-    return ci.cbase[(int64_t)slice * kSlots + s] + (int)(row & 63);
+    return base[s] + (int)(row & 63);
   }
 }
 
 template <int V>
 using cmode = std::integral_constant<int, V>;
 
-// Runs f(cmode<CM>{}) with the column mode of the row's slice.  CI: the
-// launch reads compact indices.  UNI: the slice is the same for the whole
-// wave, which keeps the choice a scalar branch.
+// Runs f(cmode<CM>{}, base) with the column mode of the row's slice and its
+// slot bases.  CI: the launch reads compact indices.  UNI: the slice is the
+// same for the whole wave: the bases and the form are read together from
+// the slice's record before the choice, scalar loads in one round trip
+// (small coarse steps are bound by their chain of dependent loads: reading
+// the form first and the bases after it cost one more), and the choice is a
+// scalar branch.  Otherwise each lane reads its slice's form from cform and
+// its bases from the record as the slots need them.
 template <bool CI, bool UNI, class F>
 __device__ __forceinline__ auto with_cols(const ColIdx &ci, int slice, F &&f) {
   if constexpr (!CI) {
-    return f(cmode<0>{});
+    return f(cmode<0>{}, (const int *)nullptr);
+  } else if constexpr (UNI) {
+    const int *rec = ci.cbase + (int64_t)slice * kMeta;
+    int base[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) base[s] = rec[s];
+    const int form = rec[kSlots];
+    if (form == kFormUniform) return f(cmode<2>{}, (const int *)base);
+    if (form == kFormDelta) return f(cmode<1>{}, (const int *)base);
+    return f(cmode<0>{}, (const int *)base);
   } else {
-    int form = ci.cform[slice];
-    if constexpr (UNI) form = __builtin_amdgcn_readfirstlane(form);
-    if (form == kFormUniform) return f(cmode<2>{});
-    if (form == kFormDelta) return f(cmode<1>{});
-    return f(cmode<0>{});
+    const int *rec = ci.cbase + (int64_t)slice * kMeta;
+    const int form = ci.cform[slice];
+    if (form == kFormUniform) return f(cmode<2>{}, rec);
+    if (form == kFormDelta) return f(cmode<1>{}, rec);
+    return f(cmode<0>{}, rec);
   }
 }
 
@@ -326,12 +340,12 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_spmv(
   if (row >= nrows) return;
   const size_t b = ell_base(row);
   const int slice = slice_of<true>(row);
-  y[row] = (XT)with_cols<CI, true>(ci, slice, [&](auto cm) {
+  y[row] = (XT)with_cols<CI, true>(ci, slice, [&](auto cm, const int *base) {
     constexpr int CM = decltype(cm)::value;
     ACC sum = 0;
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
-      const int c = col_at<CM, true>(ci, row, slice, b, s);
+      const int c = col_at<CM, true>(ci, row, base, b, s);
       const ACC v = ld_val<ACC>(vals + b + s * kSlice);
       sum += v * (ACC)x[c];
     }
@@ -375,13 +389,13 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
   int ds = kDiagSlot;
   if constexpr (GEN) ds = dslot[row];
   const ACC d = ld_val<ACC, FINE>(vals + b + ds * kSlice);
-  auto row_sum = [&](auto cm) {
+  auto row_sum = [&](auto cm, const int *base) {
     constexpr int CM = decltype(cm)::value;
     ACC sum = (ACC)r[row];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
       if (!((mask >> s) & 1u)) continue;
-      const int c = col_at<CM, FINE>(ci, row, slice, b, s);
+      const int c = col_at<CM, FINE>(ci, row, base, b, s);
       const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
       const ACC xc = (ACC)x[c];
       // SAFE (from x = 0 into a vector that may hold anything): a zero entry (a neighbour
@@ -448,12 +462,12 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_restr
   if (t >= count) return;
   const int64_t row = frow ? (int64_t)frow[t] : row0 + t;
   const size_t b = ell_base(row);
-  auto row_ax = [&](auto cm, int slice) {
+  auto row_ax = [&](auto cm, const int *base) {
     constexpr int CM = decltype(cm)::value;
     ACC ax = 0;
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
-      const int c = col_at<CM, true>(ci, row, slice, b, s);
+      const int c = col_at<CM, true>(ci, row, base, b, s);
       const ACC v = ld_val<ACC>(vals + b + s * kSlice);
       ax += v * (ACC)x[c];
     }
@@ -462,10 +476,10 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_restr
   ACC ax;
   if (frow) {  // rows from a list: slices differ across the wave
     const int slice = slice_of<false>(row);
-    ax = with_cols<CI, false>(ci, slice, [&](auto cm) { return row_ax(cm, slice); });
+    ax = with_cols<CI, false>(ci, slice, row_ax);
   } else {
     const int slice = slice_of<true>(row);
-    ax = with_cols<CI, true>(ci, slice, [&](auto cm) { return row_ax(cm, slice); });
+    ax = with_cols<CI, true>(ci, slice, row_ax);
   }
   ACC res = (ACC)r[row] - ax;
   if (SCALE) res = res * (ACC)sigma[t];

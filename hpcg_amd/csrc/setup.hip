@@ -180,7 +180,7 @@ static void free_level(Ctx *c, Level &L) {
   dfree(c, L.vals, m * sizeof(double));
   dfree(c, L.vals32, m * sizeof(float));
   dfree(c, L.vals16, m * sizeof(__half));
-  dfree(c, L.cbase, (size_t)(L.lay.npad / kSlice) * kSlots * sizeof(int));
+  dfree(c, L.cbase, (size_t)(L.lay.npad / kSlice) * kMeta * sizeof(int));
   dfree(c, L.cdelta, m * sizeof(int8_t));
   dfree(c, L.cform, (size_t)(L.lay.npad / kSlice));
   dfree(c, L.dinv, L.lay.npad * sizeof(float));
@@ -1078,19 +1078,21 @@ __global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__
     const int64_t d = target - base - lane;
     if (d < -128 || d > 127) wide = true;
     cdelta[b + s * kSlice] = (int8_t)(wide ? 0 : d);
-    if (lane == 0) cbase[slice * kSlots + s] = (int)base;
+    if (lane == 0) cbase[slice * kMeta + s] = (int)base;
   }
   // wide and uniform are per lane so far; the slice is wide if any lane is,
   // uniform if every lane is
   const bool any_wide = __any(wide), all_uniform = __all(uniform);
-  if (lane == 0) cform[slice] = any_wide ? kFormWide : all_uniform ? kFormUniform : kFormDelta;
+  const int form = any_wide ? kFormWide : all_uniform ? kFormUniform : kFormDelta;
+  if (lane == 0) cform[slice] = (uint8_t)form;
+  if (lane < kMeta - kSlots) cbase[slice * kMeta + kSlots + lane] = lane == 0 ? form : 0;  // the form, then padding
 }
 
 static int build_compact_cols(Ctx *c, Level &L) {
   if (L.cbase) return HPCGX_OK;
   const int64_t ns = L.lay.npad / kSlice;
   int rc;
-  if ((rc = dalloc(c, (void **)&L.cbase, (size_t)ns * kSlots * sizeof(int), false))) return rc;
+  if ((rc = dalloc(c, (void **)&L.cbase, (size_t)ns * kMeta * sizeof(int), false))) return rc;
   if ((rc = dalloc(c, (void **)&L.cdelta, (size_t)L.lay.npad * kSlots * sizeof(int8_t), false))) return rc;
   if ((rc = dalloc(c, (void **)&L.cform, (size_t)ns, false))) return rc;
   const int64_t grid = (ns + (kBlock / kSlice) - 1) / (kBlock / kSlice);
