@@ -14,7 +14,7 @@ BINARY = os.path.join(ROOT, "oracle", "_ref", "xhpcg_mi355x")
 pytestmark = pytest.mark.gpu
 
 
-def run_driver(tmp_path, n, threads=1):
+def run_driver(tmp_path, n, threads=1, dims=None):
     if not os.path.exists(BINARY):
         pytest.skip("reference driver not built (needs the reference sources: make -C oracle ref)")
     # One thread at small grids: the reference's own CG_ref (which sets the
@@ -23,7 +23,8 @@ def run_driver(tmp_path, n, threads=1):
     # (3.66e-41) is in the rounding noise, so with threads it moves from run
     # to run.  At 192^3 the target (5.3e-5) is far above it.
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    p = subprocess.run([BINARY, str(n), str(n), str(n), "--rt=0"], cwd=tmp_path, env=env,
+    dims = dims or (n, n, n)
+    p = subprocess.run([BINARY, *map(str, dims), "--rt=0"], cwd=tmp_path, env=env,
                        capture_output=True, text=True, timeout=900)
     assert p.returncode == 0, p.stderr[-2000:]
     reports = [f for f in os.listdir(tmp_path) if f.startswith("HPCG-Benchmark")]
@@ -48,6 +49,22 @@ def test_reference_driver_valid_with_dropin_hooks(tmp_path, n):
     assert kv["Reproducibility Information::Result"] == "PASSED"
     assert float(kv["Reproducibility Information::Scaled residual variance"]) == 0.0
     assert "Reference version of" not in text   # every hook is the device one
+
+
+def test_reference_driver_valid_on_a_box_grid(tmp_path):
+    """A grid with three different extents (48 x 32 x 24): the setup hooks,
+    the host check that recognises the HPCG rows, the device generation from
+    the closed form and the injection map all see nx != ny != nz."""
+    text, kv = run_driver(tmp_path, 0, dims=(48, 32, 24))
+    assert "HPCG result is VALID" in text, text[-3000:]
+    assert kv["Global Problem Dimensions::Global nx"] == "48"
+    assert kv["Global Problem Dimensions::Global nz"] == "24"
+    for key in ("Spectral Convergence Tests::Result", "Iteration Count Information::Result",
+                "Reproducibility Information::Result",
+                "Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon::Result"):
+        assert kv[key] == "PASSED", key
+    assert int(kv["Iteration Count Information::Optimized CG iterations per set"]) <= 52
+    assert "Reference version of" not in text
 
 
 @pytest.mark.timeout(900)
