@@ -60,6 +60,8 @@ REF_REDUCTION = {16: 3.66234e-41, 192: 5.31511e-05}
 PROFILED_TRAFFIC = {
     (192, "compact"): (269098738, "profiles/k_gs_fine_192_fp64_pmc_traffic.md"),
     (192, "explicit"): (361617890, "profiles/k_gs_fine_192_fp64_pmc_traffic.md"),
+    (512, "compact"): (5040454534, "profiles/k_gs_fine_192_fp64_pmc_traffic.md"),
+    (512, "explicit"): (6794418562, "profiles/k_gs_fine_192_fp64_pmc_traffic.md"),
 }
 
 
@@ -293,7 +295,11 @@ def leg_512(stream, g=512, reps=1):
                                 "achieved_gb_s": round(mb / nl / tl / 1e9, 1),
                                 "frac": round(mb / nl / tl / (HBM_PEAK_GBS * 1e9), 4),
                                 "as_built_gb_s": round(bb / nl / tl / 1e9, 1),
-                                "as_built_frac": round(bb / nl / tl / (HBM_PEAK_GBS * 1e9), 4)},
+                                "as_built_frac": round(bb / nl / tl / (HBM_PEAK_GBS * 1e9), 4),
+                                "traffic": PROFILED_TRAFFIC[(g, "compact")][0] if (g, "compact") in PROFILED_TRAFFIC
+                                else None,
+                                "traffic_source": PROFILED_TRAFFIC[(g, "compact")][1]
+                                if (g, "compact") in PROFILED_TRAFFIC else None},
                "column_forms_level0": forms[0]}
         log(f"[bench {g}^3] FP64 set {el * 1e3:.1f} ms ({fl / el / 1e9:.1f} GFLOP/s); fine step {tl * 1e6:.1f} us")
         out["time_to_rel_res_1e-9"] = tolerance_solves(ctx, db, dx, g, reps)
