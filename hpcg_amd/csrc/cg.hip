@@ -67,14 +67,15 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(double, CI) void k_c
   if (row < nrows) {
     const size_t b = ell_base(row);
     const int slice = slice_of<true>(row);
+    double v[kSlots];
+    row_vals<double, true, kAllSlots>(vals, row, v);
     const double sum = with_cols<CI, true>(ci, slice, [&](auto cm, const int *base) {
       constexpr int CM = decltype(cm)::value;
       double acc_row = 0.0;
 #pragma unroll
       for (int s = 0; s < kSlots; ++s) {
         const int c = col_at<CM, true>(ci, row, base, b, s);
-        const double v = ld_val<double>(vals + b + s * kSlice);
-        acc_row += v * p[c];
+        acc_row += v[s] * p[c];
       }
       return acc_row;
     });

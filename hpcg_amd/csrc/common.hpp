@@ -37,6 +37,93 @@ enum ColForm : uint8_t {
 // form and bases with scalar loads from one record, in one round trip.
 constexpr int kMeta = 32;
 
+// Value layout of a slice (27 x 64 entries).  The slots are stored in 14
+// units: 13 pairs and the diagonal alone.  Within a pair the two values of a
+// row are adjacent, so a wave reads both slots of its 64 rows with one
+// 16-byte load per lane (fp64; 8 bytes fp32, 4 bytes fp16): wide loads
+// stream from HBM faster than twice as many narrow ones.  The two slots of
+// a pair are offsets of the same parity in x, y and z, so their neighbours
+// lie in the same colour block: a step that skips the slots of some blocks
+// skips whole pairs.  Column indices keep one 64-entry run per slot.  The
+// order in which a row's products are summed is the slot order, whatever the
+// storage order.
+constexpr int kUnits = 14;
+struct ValUnit { int a, b; };  // slots of a unit; b < 0: a single slot
+__host__ __device__ constexpr ValUnit val_unit(int u) {
+  constexpr ValUnit t[kUnits] = {{0, 2},   {1, 7},   {3, 5},   {4, 22},  {6, 8},   {9, 11},  {10, 16},
+                                 {12, 14}, {13, -1}, {15, 17}, {18, 20}, {19, 25}, {21, 23}, {24, 26}};
+  return t[u];
+}
+// Offset of unit u from the start of its slice, in values.
+__host__ __device__ constexpr int val_unit_off(int u) {
+  int off = 0;
+  for (int k = 0; k < u; ++k) off += val_unit(k).b < 0 ? kSlice : 2 * kSlice;
+  return off;
+}
+// Per slot one byte: bits 0-4 the unit's offset in runs of 64 values, bit 5
+// the second slot of a pair, bit 6 a paired slot.  Packed 8 to a word so
+// that val_pos is arithmetic on constants, on the host and on the device,
+// for a slot known at compile time or not.
+__host__ __device__ constexpr uint64_t val_code_word(int w) {
+  uint64_t word = 0;
+  for (int u = 0; u < kUnits; ++u) {
+    const ValUnit v = val_unit(u);
+    const uint64_t run = (uint64_t)(val_unit_off(u) / kSlice);
+    const int sl[2] = {v.a, v.b};
+    for (int h = 0; h < 2; ++h) {
+      const int s = sl[h];
+      if (s < 0 || (s >> 3) != w) continue;
+      const uint64_t code = run | ((uint64_t)h << 5) | ((uint64_t)(v.b >= 0) << 6);
+      word |= code << ((s & 7) * 8);
+    }
+  }
+  return word;
+}
+template <int W>
+struct ValCodeWord {
+  static constexpr uint64_t v = val_code_word(W);
+};
+__host__ __device__ constexpr uint32_t val_code(int s) {
+  const uint64_t w = s < 8 ? ValCodeWord<0>::v : s < 16 ? ValCodeWord<1>::v : s < 24 ? ValCodeWord<2>::v
+                                                                                     : ValCodeWord<3>::v;
+  return (uint32_t)((w >> ((s & 7) * 8)) & 0xff);
+}
+// Index of (row, slot s) in a level's value array (fp64, fp32 and fp16 alike).
+__host__ __device__ constexpr size_t val_pos(int64_t row, int s) {
+  const uint32_t c = val_code(s);
+  const size_t lane = (size_t)(row & 63);
+  return (size_t)(row >> 6) * (kSlots * kSlice) + (size_t)(c & 31) * kSlice +
+         (((c >> 6) & 1) ? 2 * lane + ((c >> 5) & 1) : lane);
+}
+
+// Layout checks: every slot in exactly one unit, pairs of one parity class,
+// every pair at an even offset (aligned wide loads), 27 x 64 values in all.
+constexpr bool val_layout_ok() {
+  int seen[kSlots] = {};
+  for (int u = 0; u < kUnits; ++u) {
+    const ValUnit v = val_unit(u);
+    if (v.a < 0 || v.a >= kSlots || ++seen[v.a] != 1) return false;
+    if (v.b >= 0) {
+      if (v.b >= kSlots || ++seen[v.b] != 1) return false;
+      const auto odd = [](int s) { return ((s % 3) != 1) | ((((s / 3) % 3) != 1) << 1) | (((s / 9) != 1) << 2); };
+      if (odd(v.a) != odd(v.b) || val_unit_off(u) % 2) return false;
+    } else if (v.a != kDiagSlot) {
+      return false;
+    }
+  }
+  if (val_unit_off(kUnits) != kSlots * kSlice) return false;
+  for (int u = 0; u < kUnits; ++u) {
+    const ValUnit v = val_unit(u);
+    for (int lane = 0; lane < kSlice; lane += 63) {
+      const size_t o = (size_t)val_unit_off(u);
+      if (v.b < 0 && val_pos(lane, v.a) != o + lane) return false;
+      if (v.b >= 0 && (val_pos(lane, v.a) != o + 2 * lane || val_pos(lane, v.b) != o + 2 * lane + 1)) return false;
+    }
+  }
+  return true;
+}
+static_assert(val_layout_ok(), "value layout");
+
 // Column index format of a context (hpcgx_set_column_format).
 enum ColFormat : int { kColsExplicit = 0, kColsCompact = 1, kColsCompactAlways = 2 };
 

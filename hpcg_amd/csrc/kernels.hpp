@@ -3,11 +3,11 @@
 // Matrix storage: 27-slot sliced ELL.  Slot s of a row is the stencil offset
 // (dx,dy,dz) = (s%3-1, s/3%3-1, s/9-1); a neighbour outside the grid is a
 // zero value whose explicit column is the row itself.  Within a slice of 64
-// rows the layout is slot-major, so one wavefront reads each slot of its 64
-// rows as one contiguous 512-byte (fp64 value) run, plus a 256-byte run of
-// explicit indices or a 64-byte run of int8 deltas, or no per-entry index at
-// all (ColForm).  Slot order is
-// ascending column order, the order in which the reference sums a row
+// rows one wavefront reads the values of a pair of slots of its 64 rows as
+// one contiguous 1024-byte (fp64) run, 16 bytes per lane (val_pos), and each
+// slot's column as a 256-byte run of explicit indices, a 64-byte run of int8
+// deltas, or no per-entry index at all (ColForm).  Rows are summed in slot
+// order, ascending column order, the order in which the reference sums a row
 // (ComputeSPMV_ref.cpp:67-68); the zero slots add exact zeros, so sums match
 // the reference bit for bit.  The whole library is compiled with
 // -ffp-contract=off: products and sums are rounded separately, as in the
@@ -15,6 +15,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 
@@ -99,6 +100,64 @@ __device__ __forceinline__ ACC ld_val(const __half *p) {
   if constexpr (NT) bits = __builtin_nontemporal_load(q);
   else bits = *q;
   return (ACC)__half2float(__ushort_as_half(bits));
+}
+
+// Both values of a pair unit of a row (val_pos): one 16-byte load (fp64),
+// 8 bytes (fp32) or 4 bytes (fp16) per lane.
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+template <class ACC, bool NT>
+__device__ __forceinline__ void ld_pair(const double *p, ACC &a, ACC &b) {
+  const f64x2_t *q = reinterpret_cast<const f64x2_t *>(p);
+  f64x2_t v;
+  if constexpr (NT) v = __builtin_nontemporal_load(q);
+  else v = *q;
+  a = (ACC)v.x;
+  b = (ACC)v.y;
+}
+template <class ACC, bool NT>
+__device__ __forceinline__ void ld_pair(const float *p, ACC &a, ACC &b) {
+  const f32x2_t *q = reinterpret_cast<const f32x2_t *>(p);
+  f32x2_t v;
+  if constexpr (NT) v = __builtin_nontemporal_load(q);
+  else v = *q;
+  a = (ACC)v.x;
+  b = (ACC)v.y;
+}
+template <class ACC, bool NT>
+__device__ __forceinline__ void ld_pair(const __half *p, ACC &a, ACC &b) {
+  const unsigned *q = reinterpret_cast<const unsigned *>(p);
+  unsigned bits;
+  if constexpr (NT) bits = __builtin_nontemporal_load(q);
+  else bits = *q;
+  a = (ACC)__half2float(__ushort_as_half((unsigned short)(bits & 0xffffu)));
+  b = (ACC)__half2float(__ushort_as_half((unsigned short)(bits >> 16)));
+}
+
+template <class F, int... U>
+__device__ __forceinline__ void for_each_unit(F &&f, std::integer_sequence<int, U...>) {
+  (f(std::integral_constant<int, U>{}), ...);
+}
+
+// The values of the slots in MASK of a row, into v[slot]; other entries of v
+// are left unset.  A pair unit is read whole when either slot is wanted.
+constexpr uint32_t kAllSlots = (1u << kSlots) - 1;
+template <class ACC, bool NT, uint32_t MASK, class VT>
+__device__ __forceinline__ void row_vals(const VT *__restrict__ vals, int64_t row, ACC (&v)[kSlots]) {
+  const VT *sb = vals + (size_t)(row >> 6) * (kSlots * kSlice);
+  const int lane = (int)(row & 63);
+  for_each_unit(
+      [&](auto uc) {
+        constexpr int u = decltype(uc)::value;
+        constexpr ValUnit w = val_unit(u);
+        constexpr int off = val_unit_off(u);
+        if constexpr (w.b < 0) {
+          if constexpr ((MASK >> w.a) & 1u) v[w.a] = ld_val<ACC, NT>(sb + off + lane);
+        } else if constexpr (((MASK >> w.a) | (MASK >> w.b)) & 1u) {
+          ld_pair<ACC, NT>(sb + off + 2 * lane, v[w.a], v[w.b]);
+        }
+      },
+      std::make_integer_sequence<int, kUnits>{});
 }
 
 // Column index of (row, slot): the explicit 32-bit index, or the compact
@@ -340,14 +399,15 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_spmv(
   if (row >= nrows) return;
   const size_t b = ell_base(row);
   const int slice = slice_of<true>(row);
+  ACC v[kSlots];
+  row_vals<ACC, true, kAllSlots>(vals, row, v);
   y[row] = (XT)with_cols<CI, true>(ci, slice, [&](auto cm, const int *base) {
     constexpr int CM = decltype(cm)::value;
     ACC sum = 0;
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
       const int c = col_at<CM, true>(ci, row, base, b, s);
-      const ACC v = ld_val<ACC>(vals + b + s * kSlice);
-      sum += v * (ACC)x[c];
+      sum += v[s] * (ACC)x[c];
     }
     return sum;
   });
@@ -366,6 +426,18 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
   return m;
 }
 
+// The masks keep or skip the two slots of a pair together (each pair's
+// neighbours lie in one colour block).
+constexpr bool masks_keep_pairs() {
+  for (int k = 0; k < kColours; ++k)
+    for (int u = 0; u < kUnits; ++u) {
+      const ValUnit w = val_unit(u);
+      if (w.b >= 0 && ((swept_before_mask(k) >> w.a) & 1u) != ((swept_before_mask(k) >> w.b) & 1u)) return false;
+    }
+  return true;
+}
+static_assert(masks_keep_pairs(), "zero-start masks split a pair unit");
+
 // One Gauss-Seidel step on the rows of one colour block; ComputeSYMGS_ref.cpp:72-79:
 // sum = r_i - sum_j a_ij x_j (diagonal included) + x_i a_ii ; x_i = sum / a_ii.
 // ZK >= 0: the step of block ZK in the first forward sweep from x = 0 -- only
@@ -383,12 +455,14 @@ template <class VT, class XT, class ACC, int ZK, bool FINE, bool CI, bool GEN, b
 __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict__ vals,
                                          const XT *__restrict__ r, XT *__restrict__ x, int64_t row,
                                          const uint8_t *__restrict__ dslot) {
-  constexpr uint32_t mask = ZK >= 0 ? swept_before_mask(ZK) : 0x7ffffffu;
+  constexpr uint32_t mask = ZK >= 0 ? swept_before_mask(ZK) : kAllSlots;
   const size_t b = ell_base(row);
   const int slice = slice_of<true>(row);
-  int ds = kDiagSlot;
-  if constexpr (GEN) ds = dslot[row];
-  const ACC d = ld_val<ACC, FINE>(vals + b + ds * kSlice);
+  ACC vr[kSlots];
+  row_vals<ACC, FINE, mask>(vals, row, vr);
+  ACC d;
+  if constexpr (!GEN && ((mask >> kDiagSlot) & 1u)) d = vr[kDiagSlot];
+  else d = ld_val<ACC, FINE>(vals + val_pos(row, GEN ? (int)dslot[row] : kDiagSlot));
   auto row_sum = [&](auto cm, const int *base) {
     constexpr int CM = decltype(cm)::value;
     ACC sum = (ACC)r[row];
@@ -396,7 +470,7 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
     for (int s = 0; s < kSlots; ++s) {
       if (!((mask >> s) & 1u)) continue;
       const int c = col_at<CM, FINE>(ci, row, base, b, s);
-      const ACC v = ld_val<ACC, FINE>(vals + b + s * kSlice);
+      const ACC v = vr[s];
       const ACC xc = (ACC)x[c];
       // SAFE (from x = 0 into a vector that may hold anything): a zero entry (a neighbour
       // outside the grid) may point at a row not written yet in this sweep,
@@ -462,14 +536,15 @@ __global__ __launch_bounds__(kBlock) HPCGX_SPARSE_OCCUPANCY(VT, CI) void k_restr
   if (t >= count) return;
   const int64_t row = frow ? (int64_t)frow[t] : row0 + t;
   const size_t b = ell_base(row);
+  ACC v[kSlots];
+  row_vals<ACC, true, kAllSlots>(vals, row, v);
   auto row_ax = [&](auto cm, const int *base) {
     constexpr int CM = decltype(cm)::value;
     ACC ax = 0;
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
       const int c = col_at<CM, true>(ci, row, base, b, s);
-      const ACC v = ld_val<ACC>(vals + b + s * kSlice);
-      ax += v * (ACC)x[c];
+      ax += v[s] * (ACC)x[c];
     }
     return ax;
   };
