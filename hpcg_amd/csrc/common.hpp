@@ -37,92 +37,108 @@ enum ColForm : uint8_t {
 // form and bases with scalar loads from one record, in one round trip.
 constexpr int kMeta = 32;
 
-// Value layout of a slice (27 x 64 entries).  The slots are stored in 14
-// units: 13 pairs and the diagonal alone.  Within a pair the two values of a
-// row are adjacent, so a wave reads both slots of its 64 rows with one
-// 16-byte load per lane (fp64; 8 bytes fp32, 4 bytes fp16): wide loads
-// stream from HBM faster than twice as many narrow ones.  The two slots of
-// a pair are offsets of the same parity in x, y and z, so their neighbours
-// lie in the same colour block: a step that skips the slots of some blocks
-// skips whole pairs.  Column indices keep one 64-entry run per slot.  The
-// order in which a row's products are summed is the slot order, whatever the
-// storage order.
-constexpr int kUnits = 14;
-struct ValUnit { int a, b; };  // slots of a unit; b < 0: a single slot
-__host__ __device__ constexpr ValUnit val_unit(int u) {
-  constexpr ValUnit t[kUnits] = {{0, 2},   {1, 7},   {3, 5},   {4, 22},  {6, 8},   {9, 11},  {10, 16},
-                                 {12, 14}, {13, -1}, {15, 17}, {18, 20}, {19, 25}, {21, 23}, {24, 26}};
-  return t[u];
+// Value layouts of a slice (27 x 64 entries).  The slots are stored in
+// units of 1, 2 or 4 slots; within a unit the values of a row are adjacent,
+// so a wave reads a unit of its 64 rows with one load of 2 or 4 values per
+// lane: wide loads stream from HBM faster than as many narrow ones.
+//   W = 2 (fp64 values): 13 pairs and the diagonal, 16 bytes per lane;
+//   W = 4 (the fp32 and fp16 copies): 5 quads, 3 pairs and the diagonal,
+//         16 (fp32) or 8 (fp16) bytes per lane for a quad.
+// The slots of a unit are offsets of the same parity in x, y and z, so their
+// neighbours lie in the same colour block: a step that skips the slots of
+// some blocks skips whole units.  Column indices keep one 64-entry run per
+// slot.  A row's products are summed in slot order whatever the storage
+// order.
+struct ValUnit {
+  int n;     // slots in the unit
+  int s[4];  // the slots
+};
+__host__ __device__ constexpr int val_units(int W) { return W == 2 ? 14 : 9; }
+__host__ __device__ constexpr ValUnit val_unit(int W, int u) {
+  constexpr ValUnit pairs[14] = {{2, {0, 2}},   {2, {1, 7}},   {2, {3, 5}},   {2, {4, 22}},  {2, {6, 8}},
+                                 {2, {9, 11}},  {2, {10, 16}}, {2, {12, 14}}, {1, {13}},     {2, {15, 17}},
+                                 {2, {18, 20}}, {2, {19, 25}}, {2, {21, 23}}, {2, {24, 26}}};
+  constexpr ValUnit quads[9] = {{4, {0, 2, 6, 8}},    {4, {18, 20, 24, 26}}, {4, {9, 11, 15, 17}},
+                                {4, {3, 5, 21, 23}},  {4, {1, 7, 19, 25}},   {2, {12, 14}},
+                                {2, {10, 16}},        {2, {4, 22}},          {1, {13}}};
+  return W == 2 ? pairs[u] : quads[u];
 }
 // Offset of unit u from the start of its slice, in values.
-__host__ __device__ constexpr int val_unit_off(int u) {
+__host__ __device__ constexpr int val_unit_off(int W, int u) {
   int off = 0;
-  for (int k = 0; k < u; ++k) off += val_unit(k).b < 0 ? kSlice : 2 * kSlice;
+  for (int k = 0; k < u; ++k) off += val_unit(W, k).n * kSlice;
   return off;
 }
-// Per slot one byte: bits 0-4 the unit's offset in runs of 64 values, bit 5
-// the second slot of a pair, bit 6 a paired slot.  Packed 8 to a word so
-// that val_pos is arithmetic on constants, on the host and on the device,
-// for a slot known at compile time or not.
-__host__ __device__ constexpr uint64_t val_code_word(int w) {
-  uint64_t word = 0;
-  for (int u = 0; u < kUnits; ++u) {
-    const ValUnit v = val_unit(u);
-    const uint64_t run = (uint64_t)(val_unit_off(u) / kSlice);
-    const int sl[2] = {v.a, v.b};
-    for (int h = 0; h < 2; ++h) {
-      const int s = sl[h];
-      if (s < 0 || (s >> 3) != w) continue;
-      const uint64_t code = run | ((uint64_t)h << 5) | ((uint64_t)(v.b >= 0) << 6);
-      word |= code << ((s & 7) * 8);
+// Per slot 9 bits: bits 0-4 the unit's offset in runs of 64 values, bits
+// 5-6 the slot's place in its unit, bits 7-8 log2 of the unit's size.
+// Packed 7 to a word so that val_pos is arithmetic on constants, on the host
+// and on the device, for a slot known at compile time or not.
+__host__ __device__ constexpr uint64_t val_code_word(int W, int word) {
+  uint64_t code = 0;
+  for (int u = 0; u < val_units(W); ++u) {
+    const ValUnit v = val_unit(W, u);
+    const uint64_t run = (uint64_t)(val_unit_off(W, u) / kSlice), lw = v.n == 4 ? 2 : v.n == 2 ? 1 : 0;
+    for (int h = 0; h < v.n; ++h) {
+      const int s = v.s[h];
+      if (s / 7 != word) continue;
+      code |= (run | ((uint64_t)h << 5) | (lw << 7)) << ((s % 7) * 9);
     }
   }
-  return word;
+  return code;
 }
-template <int W>
+template <int W, int K>
 struct ValCodeWord {
-  static constexpr uint64_t v = val_code_word(W);
+  static constexpr uint64_t v = val_code_word(W, K);
 };
+template <int W>
 __host__ __device__ constexpr uint32_t val_code(int s) {
-  const uint64_t w = s < 8 ? ValCodeWord<0>::v : s < 16 ? ValCodeWord<1>::v : s < 24 ? ValCodeWord<2>::v
-                                                                                     : ValCodeWord<3>::v;
-  return (uint32_t)((w >> ((s & 7) * 8)) & 0xff);
+  const int k = s / 7;
+  const uint64_t w = k == 0 ? ValCodeWord<W, 0>::v : k == 1 ? ValCodeWord<W, 1>::v : k == 2 ? ValCodeWord<W, 2>::v
+                                                                                             : ValCodeWord<W, 3>::v;
+  return (uint32_t)((w >> ((s - 7 * k) * 9)) & 0x1ffu);
 }
-// Index of (row, slot s) in a level's value array (fp64, fp32 and fp16 alike).
+// Index of (row, slot s) in a value array of layout W.
+template <int W>
 __host__ __device__ constexpr size_t val_pos(int64_t row, int s) {
-  const uint32_t c = val_code(s);
-  const size_t lane = (size_t)(row & 63);
+  const uint32_t c = val_code<W>(s);
   return (size_t)(row >> 6) * (kSlots * kSlice) + (size_t)(c & 31) * kSlice +
-         (((c >> 6) & 1) ? 2 * lane + ((c >> 5) & 1) : lane);
+         ((size_t)(row & 63) << ((c >> 7) & 3)) + ((c >> 5) & 3);
 }
+// Layout of the value array of element type VT.
+template <class VT>
+constexpr int val_width = sizeof(VT) == 8 ? 2 : 4;
 
-// Layout checks: every slot in exactly one unit, pairs of one parity class,
-// every pair at an even offset (aligned wide loads), 27 x 64 values in all.
+// Layout checks: every slot in exactly one unit, each unit within one parity
+// class, units aligned to their load size, 27 x 64 values in all, and
+// val_pos agreeing with the unit table.
+template <int W>
 constexpr bool val_layout_ok() {
   int seen[kSlots] = {};
-  for (int u = 0; u < kUnits; ++u) {
-    const ValUnit v = val_unit(u);
-    if (v.a < 0 || v.a >= kSlots || ++seen[v.a] != 1) return false;
-    if (v.b >= 0) {
-      if (v.b >= kSlots || ++seen[v.b] != 1) return false;
-      const auto odd = [](int s) { return ((s % 3) != 1) | ((((s / 3) % 3) != 1) << 1) | (((s / 9) != 1) << 2); };
-      if (odd(v.a) != odd(v.b) || val_unit_off(u) % 2) return false;
-    } else if (v.a != kDiagSlot) {
-      return false;
+  const auto odd = [](int s) { return ((s % 3) != 1) | ((((s / 3) % 3) != 1) << 1) | (((s / 9) != 1) << 2); };
+  for (int u = 0; u < val_units(W); ++u) {
+    const ValUnit v = val_unit(W, u);
+    if (v.n != 1 && v.n != 2 && v.n != 4) return false;
+    if (v.n == 4 && W != 4) return false;
+    if (val_unit_off(W, u) % v.n) return false;
+    for (int h = 0; h < v.n; ++h) {
+      if (v.s[h] < 0 || v.s[h] >= kSlots || ++seen[v.s[h]] != 1) return false;
+      if (odd(v.s[h]) != odd(v.s[0])) return false;
     }
+    if (v.n == 1 && v.s[0] != kDiagSlot) return false;
   }
-  if (val_unit_off(kUnits) != kSlots * kSlice) return false;
-  for (int u = 0; u < kUnits; ++u) {
-    const ValUnit v = val_unit(u);
-    for (int lane = 0; lane < kSlice; lane += 63) {
-      const size_t o = (size_t)val_unit_off(u);
-      if (v.b < 0 && val_pos(lane, v.a) != o + lane) return false;
-      if (v.b >= 0 && (val_pos(lane, v.a) != o + 2 * lane || val_pos(lane, v.b) != o + 2 * lane + 1)) return false;
-    }
+  if (val_unit_off(W, val_units(W)) != kSlots * kSlice) return false;
+  for (int u = 0; u < val_units(W); ++u) {
+    const ValUnit v = val_unit(W, u);
+    for (int64_t row = 0; row < 2 * kSlice; row += 63)
+      for (int h = 0; h < v.n; ++h) {
+        const size_t want = (size_t)(row >> 6) * (kSlots * kSlice) + (size_t)val_unit_off(W, u) +
+                            (size_t)(row & 63) * v.n + h;
+        if (val_pos<W>(row, v.s[h]) != want) return false;
+      }
   }
   return true;
 }
-static_assert(val_layout_ok(), "value layout");
+static_assert(val_layout_ok<2>() && val_layout_ok<4>(), "value layouts");
 
 // Column index format of a context (hpcgx_set_column_format).
 enum ColFormat : int { kColsExplicit = 0, kColsCompact = 1, kColsCompactAlways = 2 };

@@ -3,10 +3,11 @@
 // Matrix storage: 27-slot sliced ELL.  Slot s of a row is the stencil offset
 // (dx,dy,dz) = (s%3-1, s/3%3-1, s/9-1); a neighbour outside the grid is a
 // zero value whose explicit column is the row itself.  Within a slice of 64
-// rows one wavefront reads the values of a pair of slots of its 64 rows as
-// one contiguous 1024-byte (fp64) run, 16 bytes per lane (val_pos), and each
-// slot's column as a 256-byte run of explicit indices, a 64-byte run of int8
-// deltas, or no per-entry index at all (ColForm).  Rows are summed in slot
+// rows one wavefront reads the fp64 values of a pair of slots of its 64 rows
+// as one contiguous 1024-byte run, 16 bytes per lane (the fp32 and fp16
+// copies: four slots per load; val_pos), and each slot's column as a
+// 256-byte run of explicit indices, a 64-byte run of int8 deltas, or no
+// per-entry index at all (ColForm).  Rows are summed in slot
 // order, ascending column order, the order in which the reference sums a row
 // (ComputeSPMV_ref.cpp:67-68); the zero slots add exact zeros, so sums match
 // the reference bit for bit.  The whole library is compiled with
@@ -102,36 +103,59 @@ __device__ __forceinline__ ACC ld_val(const __half *p) {
   return (ACC)__half2float(__ushort_as_half(bits));
 }
 
-// Both values of a pair unit of a row (val_pos): one 16-byte load (fp64),
-// 8 bytes (fp32) or 4 bytes (fp16) per lane.
+// The values of a unit of a row (val_pos): 2 or 4 adjacent values read with
+// one load per lane -- a pair of fp64 values in 16 bytes, a quad of fp32 in
+// 16, a quad of fp16 in 8, a pair of fp32 or fp16 in 8 or 4.
 typedef double f64x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
-template <class ACC, bool NT>
-__device__ __forceinline__ void ld_pair(const double *p, ACC &a, ACC &b) {
-  const f64x2_t *q = reinterpret_cast<const f64x2_t *>(p);
-  f64x2_t v;
-  if constexpr (NT) v = __builtin_nontemporal_load(q);
-  else v = *q;
-  a = (ACC)v.x;
-  b = (ACC)v.y;
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+template <class V, bool NT>
+__device__ __forceinline__ V ld_vec(const V *q) {
+  if constexpr (NT) return __builtin_nontemporal_load(q);
+  else return *q;
+}
+template <class ACC>
+__device__ __forceinline__ ACC half_lo(unsigned bits) {
+  return (ACC)__half2float(__ushort_as_half((unsigned short)(bits & 0xffffu)));
+}
+template <class ACC>
+__device__ __forceinline__ ACC half_hi(unsigned bits) {
+  return (ACC)__half2float(__ushort_as_half((unsigned short)(bits >> 16)));
 }
 template <class ACC, bool NT>
-__device__ __forceinline__ void ld_pair(const float *p, ACC &a, ACC &b) {
-  const f32x2_t *q = reinterpret_cast<const f32x2_t *>(p);
-  f32x2_t v;
-  if constexpr (NT) v = __builtin_nontemporal_load(q);
-  else v = *q;
-  a = (ACC)v.x;
-  b = (ACC)v.y;
+__device__ __forceinline__ void ld_unit(const double *p, ACC *a, std::integral_constant<int, 2>) {
+  const f64x2_t v = ld_vec<f64x2_t, NT>(reinterpret_cast<const f64x2_t *>(p));
+  a[0] = (ACC)v.x;
+  a[1] = (ACC)v.y;
 }
 template <class ACC, bool NT>
-__device__ __forceinline__ void ld_pair(const __half *p, ACC &a, ACC &b) {
-  const unsigned *q = reinterpret_cast<const unsigned *>(p);
-  unsigned bits;
-  if constexpr (NT) bits = __builtin_nontemporal_load(q);
-  else bits = *q;
-  a = (ACC)__half2float(__ushort_as_half((unsigned short)(bits & 0xffffu)));
-  b = (ACC)__half2float(__ushort_as_half((unsigned short)(bits >> 16)));
+__device__ __forceinline__ void ld_unit(const float *p, ACC *a, std::integral_constant<int, 2>) {
+  const f32x2_t v = ld_vec<f32x2_t, NT>(reinterpret_cast<const f32x2_t *>(p));
+  a[0] = (ACC)v.x;
+  a[1] = (ACC)v.y;
+}
+template <class ACC, bool NT>
+__device__ __forceinline__ void ld_unit(const float *p, ACC *a, std::integral_constant<int, 4>) {
+  const f32x4_t v = ld_vec<f32x4_t, NT>(reinterpret_cast<const f32x4_t *>(p));
+  a[0] = (ACC)v.x;
+  a[1] = (ACC)v.y;
+  a[2] = (ACC)v.z;
+  a[3] = (ACC)v.w;
+}
+template <class ACC, bool NT>
+__device__ __forceinline__ void ld_unit(const __half *p, ACC *a, std::integral_constant<int, 2>) {
+  const unsigned bits = ld_vec<unsigned, NT>(reinterpret_cast<const unsigned *>(p));
+  a[0] = half_lo<ACC>(bits);
+  a[1] = half_hi<ACC>(bits);
+}
+template <class ACC, bool NT>
+__device__ __forceinline__ void ld_unit(const __half *p, ACC *a, std::integral_constant<int, 4>) {
+  const u32x2_t bits = ld_vec<u32x2_t, NT>(reinterpret_cast<const u32x2_t *>(p));
+  a[0] = half_lo<ACC>(bits.x);
+  a[1] = half_hi<ACC>(bits.x);
+  a[2] = half_lo<ACC>(bits.y);
+  a[3] = half_hi<ACC>(bits.y);
 }
 
 template <class F, int... U>
@@ -139,25 +163,38 @@ __device__ __forceinline__ void for_each_unit(F &&f, std::integer_sequence<int, 
   (f(std::integral_constant<int, U>{}), ...);
 }
 
+// Slots of a unit as a mask.
+__host__ __device__ constexpr uint32_t unit_mask(const ValUnit &w) {
+  uint32_t m = 0;
+  for (int h = 0; h < w.n; ++h) m |= 1u << w.s[h];
+  return m;
+}
+
 // The values of the slots in MASK of a row, into v[slot]; other entries of v
-// are left unset.  A pair unit is read whole when either slot is wanted.
+// are left unset.  A unit is read whole when any of its slots is wanted.
 constexpr uint32_t kAllSlots = (1u << kSlots) - 1;
 template <class ACC, bool NT, uint32_t MASK, class VT>
 __device__ __forceinline__ void row_vals(const VT *__restrict__ vals, int64_t row, ACC (&v)[kSlots]) {
+  constexpr int W = val_width<VT>;
   const VT *sb = vals + (size_t)(row >> 6) * (kSlots * kSlice);
   const int lane = (int)(row & 63);
   for_each_unit(
       [&](auto uc) {
         constexpr int u = decltype(uc)::value;
-        constexpr ValUnit w = val_unit(u);
-        constexpr int off = val_unit_off(u);
-        if constexpr (w.b < 0) {
-          if constexpr ((MASK >> w.a) & 1u) v[w.a] = ld_val<ACC, NT>(sb + off + lane);
-        } else if constexpr (((MASK >> w.a) | (MASK >> w.b)) & 1u) {
-          ld_pair<ACC, NT>(sb + off + 2 * lane, v[w.a], v[w.b]);
+        constexpr ValUnit w = val_unit(W, u);
+        constexpr int off = val_unit_off(W, u);
+        if constexpr ((MASK & unit_mask(w)) != 0) {
+          if constexpr (w.n == 1) {
+            v[w.s[0]] = ld_val<ACC, NT>(sb + off + lane);
+          } else {
+            ACC t[w.n];
+            ld_unit<ACC, NT>(sb + off + w.n * lane, t, std::integral_constant<int, w.n>{});
+#pragma unroll
+            for (int h = 0; h < w.n; ++h) v[w.s[h]] = t[h];
+          }
         }
       },
-      std::make_integer_sequence<int, kUnits>{});
+      std::make_integer_sequence<int, val_units(W)>{});
 }
 
 // Column index of (row, slot): the explicit 32-bit index, or the compact
@@ -426,17 +463,17 @@ __host__ __device__ constexpr uint32_t swept_before_mask(int k) {
   return m;
 }
 
-// The masks keep or skip the two slots of a pair together (each pair's
+// The masks keep or skip the slots of a unit together (each unit's
 // neighbours lie in one colour block).
-constexpr bool masks_keep_pairs() {
+constexpr bool masks_keep_units(int W) {
   for (int k = 0; k < kColours; ++k)
-    for (int u = 0; u < kUnits; ++u) {
-      const ValUnit w = val_unit(u);
-      if (w.b >= 0 && ((swept_before_mask(k) >> w.a) & 1u) != ((swept_before_mask(k) >> w.b) & 1u)) return false;
+    for (int u = 0; u < val_units(W); ++u) {
+      const uint32_t um = unit_mask(val_unit(W, u)), m = swept_before_mask(k) & um;
+      if (m != 0 && m != um) return false;
     }
   return true;
 }
-static_assert(masks_keep_pairs(), "zero-start masks split a pair unit");
+static_assert(masks_keep_units(2) && masks_keep_units(4), "zero-start masks split a value unit");
 
 // One Gauss-Seidel step on the rows of one colour block; ComputeSYMGS_ref.cpp:72-79:
 // sum = r_i - sum_j a_ij x_j (diagonal included) + x_i a_ii ; x_i = sum / a_ii.
@@ -462,7 +499,7 @@ __device__ __forceinline__ double gs_row(const ColIdx &ci, const VT *__restrict_
   row_vals<ACC, FINE, mask>(vals, row, vr);
   ACC d;
   if constexpr (!GEN && ((mask >> kDiagSlot) & 1u)) d = vr[kDiagSlot];
-  else d = ld_val<ACC, FINE>(vals + val_pos(row, GEN ? (int)dslot[row] : kDiagSlot));
+  else d = ld_val<ACC, FINE>(vals + val_pos<val_width<VT>>(row, GEN ? (int)dslot[row] : kDiagSlot));
   auto row_sum = [&](auto cm, const int *base) {
     constexpr int CM = decltype(cm)::value;
     ACC sum = (ACC)r[row];

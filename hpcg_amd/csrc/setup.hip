@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, 
     iperm[p] = -1;
     for (int s = 0; s < kSlots; ++s) {
       cols[b + s * kSlice] = (int)p;
-      vals[val_pos(p, s)] = s == kDiagSlot ? 1.0 : 0.0;
+      vals[val_pos<2>(p, s)] = s == kDiagSlot ? 1.0 : 0.0;
     }
     return;
   }
@@ -274,25 +274,25 @@ __global__ __launch_bounds__(kBlock) void k_generate(LayoutDev L, int64_t npad, 
     const bool in = jx >= 0 && jx < L.nx && jy >= 0 && jy < L.ny && jz >= 0 && jz < L.nz;
     if (!in) {
       cols[b + s * kSlice] = (int)p;
-      vals[val_pos(p, s)] = 0.0;
+      vals[val_pos<2>(p, s)] = 0.0;
       continue;
     }
     cols[b + s * kSlice] = (int)perm_of(L, jx, jy, jz);
     if (!values) {  // structure only: the uploaded values come next
-      vals[val_pos(p, s)] = 0.0;
+      vals[val_pos<2>(p, s)] = 0.0;
       continue;
     }
     if (s == kDiagSlot) continue;
     ++nn;
     if (kappa) {
       const double kj = kappa[((int64_t)jz * L.ny + jy) * L.nx + jx];
-      vals[val_pos(p, s)] = -((ki + kj) * 0.5);
+      vals[val_pos<2>(p, s)] = -((ki + kj) * 0.5);
       dsum += (ki + kj) * 0.5;
     } else {
-      vals[val_pos(p, s)] = -1.0;
+      vals[val_pos<2>(p, s)] = -1.0;
     }
   }
-  if (values) vals[val_pos(p, kDiagSlot)] = kappa ? dsum + (double)(26 - nn) * ki : 26.0;
+  if (values) vals[val_pos<2>(p, kDiagSlot)] = kappa ? dsum + (double)(26 - nn) * ki : 26.0;
 }
 
 // coarse kappa by injection (GenerateCoarseProblem.cpp:71-82 applied to kappa)
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_rows(LayoutDev L, int64_t n, co
     if (s <= last) { ok = false; break; }
     last = s;
     diag = diag || s == kDiagSlot;
-    vals[val_pos(p, s)] = svals[t * kSlots + j];
+    vals[val_pos<2>(p, s)] = svals[t * kSlots + j];
   }
   if (!ok || !diag) err[threadIdx.x & 63] = 1;
 }
@@ -735,7 +735,7 @@ static int pack_general_row(Level &L, RowAcc &rows, int64_t p, std::vector<int> 
   const int i = L.iperm_h[p];
   for (int s = 0; s < kSlots; ++s) {
     hc[b + s * kSlice] = (int)p;
-    hv[val_pos(p, s)] = (i < 0 && s == 0) ? 1.0 : 0.0;
+    hv[val_pos<2>(p, s)] = (i < 0 && s == 0) ? 1.0 : 0.0;
   }
   if (i < 0) return 0;
   const int cnt = rows.nnz(i);
@@ -744,7 +744,7 @@ static int pack_general_row(Level &L, RowAcc &rows, int64_t p, std::vector<int> 
   int ds = -1;
   for (int j = 0; j < cnt; ++j) {
     hc[b + j * kSlice] = L.perm_h[rc[j]];
-    hv[val_pos(p, j)] = rv[j];
+    hv[val_pos<2>(p, j)] = rv[j];
     if (rc[j] == i && ds < 0) ds = j;
   }
   hd[p] = (uint8_t)ds;
@@ -998,15 +998,17 @@ int ensure_ready(Ctx *c) {
 // ---------------------------------------------------------------------------
 // lower-precision copies
 
-__global__ void k_to_f32(const double *__restrict__ v, float *__restrict__ o, int64_t m) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < m) o[i] = (float)v[i];
+// fp32 copy of the values, from the fp64 layout into the fp32 one
+__global__ void k_to_f32(const double *__restrict__ v, float *__restrict__ o, int64_t npad) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npad) return;
+  for (int s = 0; s < kSlots; ++s) o[val_pos<4>(p, s)] = (float)v[val_pos<2>(p, s)];
 }
 
 // Address of the diagonal of row p: slot 13 of the stencil layout, dslot[p]
 // of a general level.
 __device__ __forceinline__ size_t diag_at(const uint8_t *dslot, int64_t p) {
-  return val_pos(p, dslot ? dslot[p] : kDiagSlot);
+  return val_pos<2>(p, dslot ? dslot[p] : kDiagSlot);
 }
 
 // dinv = 1/sqrt(a_ii) per row (padding: a_ii = 1)
@@ -1025,7 +1027,7 @@ __global__ void k_to_f16_scaled(const int *__restrict__ cols, const double *__re
   if (p >= npad) return;
   const size_t b = ell_base(p);
   const int ds = dslot ? dslot[p] : kDiagSlot;
-  const double di = 1.0 / sqrt(vals[val_pos(p, ds)]);
+  const double di = 1.0 / sqrt(vals[val_pos<2>(p, ds)]);
   for (int s = 0; s < kSlots; ++s) {
     double a;
     if (s == ds) {
@@ -1033,9 +1035,9 @@ __global__ void k_to_f16_scaled(const int *__restrict__ cols, const double *__re
     } else {
       const int cj = cols[b + s * kSlice];
       const double dj = 1.0 / sqrt(vals[diag_at(dslot, cj)]);
-      a = vals[val_pos(p, s)] * di * dj;
+      a = vals[val_pos<2>(p, s)] * di * dj;
     }
-    v16[val_pos(p, s)] = __float2half_rn((float)a);
+    v16[val_pos<4>(p, s)] = __float2half_rn((float)a);
   }
 }
 
@@ -1077,7 +1079,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_cols(const int *__restrict__
   bool wide = false, uniform = true;
   for (int s = 0; s < kSlots; ++s) {
     const int col = cols[b + s * kSlice];
-    const bool live = vals[val_pos(row, s)] != 0.0 || s == ds;
+    const bool live = vals[val_pos<2>(row, s)] != 0.0 || s == ds;
     int lo = live ? col - lane : INT_MAX, hi = live ? col - lane : INT_MIN;
     for (int m = 32; m >= 1; m >>= 1) {
       lo = min(lo, __shfl_xor(lo, m, 64));
@@ -1149,7 +1151,7 @@ static int build_precision(Ctx *c, int prec) {
     const int64_t m = L.lay.npad * kSlots;
     if (prec == HPCGX_PREC_FP32) {
       if (!L.vals32 && (rc = dalloc(c, (void **)&L.vals32, m * sizeof(float), false))) return rc;
-      k_to_f32<<<nblk(m), kBlock, 0, c->stream>>>(L.vals, L.vals32, m);
+      k_to_f32<<<nblk(L.lay.npad), kBlock, 0, c->stream>>>(L.vals, L.vals32, L.lay.npad);
       HPCGX_CHECK_LAUNCH(c);
     } else {
       if (!L.vals16 && (rc = dalloc(c, (void **)&L.vals16, m * sizeof(__half), false))) return rc;
@@ -1639,7 +1641,7 @@ int hpcgx_level_export_packed(hpcgx_ctx *c, int level, uint8_t *nnz_row, int *co
       }
       const int pc = hc[b + s * kSlice];
       cols[(int64_t)i * kSlots + k] = pc >= 0 && pc < L.lay.npad ? ip[pc] : -2;
-      vals[(int64_t)i * kSlots + k] = hv[val_pos(p, s)];
+      vals[(int64_t)i * kSlots + k] = hv[val_pos<2>(p, s)];
       ++k;
     }
     nnz_row[i] = (uint8_t)k;
