@@ -321,6 +321,7 @@ enum FinishOp : int {
 };
 
 constexpr int kFinishThreads = 1024;
+constexpr int kFinishLoads = 32;  // partials per finishing thread read in one round
 
 // Reductions over more than kFoldAbove partials first fold each run of kFold
 // consecutive partials into one (one block per run), so the one-block
@@ -357,14 +358,43 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const double *__restr
   for (int k = 0; k < NV; ++k) {
     const double *p = partials + (size_t)k * G;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int64_t i = threadIdx.x;
-    for (; i + 3 * kFinishThreads < G; i += 4 * kFinishThreads) {
-      a0 += p[i];
-      a1 += p[i + kFinishThreads];
-      a2 += p[i + 2 * kFinishThreads];
-      a3 += p[i + 3 * kFinishThreads];
+    constexpr int64_t T = kFinishThreads;
+    const int64_t t = threadIdx.x;
+    if (G <= kFinishLoads * T) {
+      // The additions of the loop below, in the same order into the same
+      // accumulators, with all loads issued first: one round trip to memory
+      // instead of one per group of four.  Element j of this thread is
+      // p[t + j T]; the loop adds it to a(j % 4) in group j / 4 while the
+      // group's last element is below G, then the rest to a0.
+      double v[kFinishLoads];
+#pragma unroll
+      for (int j = 0; j < kFinishLoads; ++j) {
+        const int64_t i = t + j * T;
+        v[j] = i < G ? p[i] : 0.0;
+      }
+      const int64_t ngroups = G > t + 3 * T ? (G - t - 3 * T - 1) / (4 * T) + 1 : 0;
+      const int64_t nmine = G > t ? (G - t - 1) / T + 1 : 0;
+#pragma unroll
+      for (int j = 0; j < kFinishLoads; ++j) {
+        if (j / 4 < ngroups) {
+          if (j % 4 == 0) a0 += v[j];
+          else if (j % 4 == 1) a1 += v[j];
+          else if (j % 4 == 2) a2 += v[j];
+          else a3 += v[j];
+        } else if (j < nmine) {
+          a0 += v[j];
+        }
+      }
+    } else {
+      int64_t i = t;
+      for (; i + 3 * T < G; i += 4 * T) {
+        a0 += p[i];
+        a1 += p[i + T];
+        a2 += p[i + 2 * T];
+        a3 += p[i + 3 * T];
+      }
+      for (; i < G; i += T) a0 += p[i];
     }
-    for (; i < G; i += kFinishThreads) a0 += p[i];
     const double s = wave_sum((a0 + a1) + (a2 + a3));
     if (lane == 0) sh[w][k] = s;
   }
