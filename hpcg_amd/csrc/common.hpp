@@ -252,7 +252,8 @@ struct Level {
   std::vector<int> iperm_h;  // colour order -> natural, -1 on padding [npad]
   std::vector<int> f2c_h;    // natural f2c of this level (uploaded levels), [n_coarse]
   std::vector<uint8_t> rowlen_h;  // general levels: entries per natural row
-  // device matrix, slot-major inside 64-row slices: (slice*27+slot)*64+lane
+  // device matrix in 64-row slices: columns slot-major, (slice*27+slot)*64+lane;
+  // values in units of slots (val_pos<2> for vals, val_pos<4> for the copies)
   int *cols = nullptr;
   double *vals = nullptr;
   float *vals32 = nullptr;
