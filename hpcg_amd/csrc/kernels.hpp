@@ -54,15 +54,16 @@ __device__ __forceinline__ int64_t tile_id() {
 // Occupancy of the fp64 sparse kernels (SpMV, Gauss-Seidel steps,
 // restriction).  With explicit 32-bit column indices: at most two waves per
 // SIMD, so a wave holds the column, value and gather loads of all its slots
-// in flight (169 VGPRs) instead of the default's batches of about 9 at six
-// waves per SIMD (measured: 3 % faster at 512^3, the CG SpMV 9 %, 1.7 % at
-// 192^3; eight waves per SIMD was 10 % slower).  With compact columns
-// (the default) a uniform slice needs no index load, all 54 loads of a row
-// still issue before the first wait at 122 VGPRs, and no limit is best:
-// measured in alternating processes, the 192^3 sets ran 1245-1249 GFLOP/s
-// unlimited, 1248-1249 at four waves per SIMD, 1241 at three and 1181 at
-// two; the 512^3 FP64 solve 37.9-38.3 ms per iteration unlimited against
-// 40.7-41.5 at two.  HPCGX_CI_FP64_WAVES overrides it at build time.  The
+// in flight instead of the default's batches of about 9 at six waves per
+// SIMD (measured with one value slot per load: 3 % faster at 512^3, the CG
+// SpMV 9 %, 1.7 % at 192^3; eight waves per SIMD was 10 % slower).  With
+// compact columns (the default) a uniform slice needs no index load, all
+// loads of a row still issue before the first wait (114 VGPRs for the fine
+// step with paired values), and no limit is best: measured in alternating
+// processes with one value slot per load, the 192^3 sets ran 1245-1249
+// GFLOP/s unlimited, 1248-1249 at four waves per SIMD, 1241 at three and
+// 1181 at two; the 512^3 FP64 solve 37.9-38.3 ms per iteration unlimited
+// against 40.7-41.5 at two.  HPCGX_CI_FP64_WAVES overrides it at build time.  The
 // fp32 / fp16 kernels keep the default, and so do steps that read fewer
 // than 8 slots (the backend pads every kernel under the limit to its
 // register budget).
