@@ -360,21 +360,21 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const double *__restr
     const double *p = partials + (size_t)k * G;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     constexpr int64_t T = kFinishThreads;
-    const int64_t t = threadIdx.x;
+    const int64_t tid = threadIdx.x;
     if (G <= kFinishLoads * T) {
       // The additions of the loop below, in the same order into the same
       // accumulators, with all loads issued first: one round trip to memory
       // instead of one per group of four.  Element j of this thread is
-      // p[t + j T]; the loop adds it to a(j % 4) in group j / 4 while the
+      // p[tid + j T]; the loop adds it to a(j % 4) in group j / 4 while the
       // group's last element is below G, then the rest to a0.
       double v[kFinishLoads];
 #pragma unroll
       for (int j = 0; j < kFinishLoads; ++j) {
-        const int64_t i = t + j * T;
+        const int64_t i = tid + j * T;
         v[j] = i < G ? p[i] : 0.0;
       }
-      const int64_t ngroups = G > t + 3 * T ? (G - t - 3 * T - 1) / (4 * T) + 1 : 0;
-      const int64_t nmine = G > t ? (G - t - 1) / T + 1 : 0;
+      const int64_t ngroups = G > tid + 3 * T ? (G - tid - 3 * T - 1) / (4 * T) + 1 : 0;
+      const int64_t nmine = G > tid ? (G - tid - 1) / T + 1 : 0;
 #pragma unroll
       for (int j = 0; j < kFinishLoads; ++j) {
         if (j / 4 < ngroups) {
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish(const double *__restr
         }
       }
     } else {
-      int64_t i = t;
+      int64_t i = tid;
       for (; i + 3 * T < G; i += 4 * T) {
         a0 += p[i];
         a1 += p[i + T];
