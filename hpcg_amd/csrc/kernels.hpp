@@ -31,8 +31,9 @@ __device__ __forceinline__ size_t ell_base(int64_t row) {
 // ROT (the fine level's fp64 Gauss-Seidel steps): XCD x starts its range at
 // x/8 of it and wraps.  A step's 8 ranges are slabs of one colour block,
 // equally spaced in memory, and swept in lockstep they meet the same HBM
-// channels at the same time; rotated, the fine 192^3 step takes 57.6 us
-// instead of 59.3 (4 alternating processes each).  Whole-level passes (SpMV)
+// channels at the same time; rotated, the fine 192^3 step took 57.6 us
+// instead of 59.3 (explicit indices, one value slot per load; 4 alternating
+// processes each).  Whole-level passes (SpMV)
 // keep the plain map: there XCD x sweeps block x, and in lockstep the 8 XCDs
 // gather x around the same grid position, which the last-level cache serves
 // once (rotated, the 512^3 SpMV took 9.2-9.6 ms instead of 8.1-8.5).  The
