@@ -26,13 +26,19 @@ static void gs_block(Ctx *c, const Level &L, const VT *v, const XT *r, XT *x, in
   const int64_t r0 = L.lay.start[k], nr = L.lay.start[k + 1] - r0;
   if (nr <= 0) return;
   const ColIdx ci = col_idx(c, L);
-  // Coarse fp64 steps of blocks up to 2^20 rows launch as one-wave blocks,
-  // which spreads them over more CUs: with explicit columns at two waves per
-  // SIMD a 192^3 set measured 2-6 % faster, 512^3 about 5 %; with compact
-  // columns and no occupancy limit, 1245 against 1219 GFLOP/s at 192^3 (512^3
-  // within its run-to-run spread).  The fine level, the other precisions and
-  // the r.z form keep 256-thread blocks.
-  const unsigned bs = (!FINE && sizeof(VT) == 8 && nr < ((int64_t)1 << 20)) ? kSlice : kBlock;
+  // Coarse fp64 steps of blocks under HPCGX_ONE_WAVE_ROWS (2^20) rows launch
+  // as one-wave blocks, which spreads them over more CUs: with explicit
+  // columns at two waves per SIMD a 192^3 set measured 2-6 % faster, 512^3
+  // about 5 %; with compact columns and no occupancy limit, 1245 against 1219
+  // GFLOP/s at 192^3 (512^3 within its run-to-run spread); with paired values,
+  // 94.6-95.4 ms per 192^3 set against 95.5-95.6 with the threshold at 2^16
+  // (the 96^3 level's steps in 256-thread blocks) and 96.1-96.8 with none.
+  // The fine level, the other precisions and the r.z form keep 256-thread
+  // blocks.
+#ifndef HPCGX_ONE_WAVE_ROWS
+#define HPCGX_ONE_WAVE_ROWS (1 << 20)
+#endif
+  const unsigned bs = (!FINE && sizeof(VT) == 8 && nr < (int64_t)(HPCGX_ONE_WAVE_ROWS)) ? kSlice : kBlock;
   const unsigned g = (unsigned)((nr + bs - 1) / bs);
   hipStream_t st = c->stream;
   if constexpr (ZK < 0 && sizeof(VT) == 8 && sizeof(XT) == 8) {
