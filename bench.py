@@ -11,7 +11,9 @@ The same JSON line also carries:
   * the rating with the setup charged as ReportResults.cpp:353 charges it
     (device generation and packing, amortised /10 per set);
   * HBM rates of the iteration on the model's bytes and on the bytes the
-    kernels load as built (zero-start masks, elided step, column forms);
+    kernels load as built (zero-start masks, elided step, column forms),
+    and the reference report's own "Raw B/W" figures (ReportResults.cpp:
+    103-135,339-341) for comparison;
   * the roofline of the dominant kernel (the fine-level Gauss-Seidel colour
     step, timed with HIP events on the stream it runs on);
   * the same set with the explicit 32-bit column indices (the reference's
@@ -27,7 +29,8 @@ The same JSON line also carries:
     (FP16 also to 1e-12), forward error against an FP64 solve at 1e-12;
   * the 128^3 adjoint through torch.autograd: backward / forward time,
     finite differences on 64 kappa entries, the FP64 gradient;
-  * the reference CPU build timed on this host on a bounded sample.
+  * the reference CPU build timed on this host on a bounded sample, with the
+    host's CPU model, threads and NUMA nodes.
 
     python bench.py [--steps K] [--warmup W] [--grid N] [--gpus 1]
                     [--no-512] [--no-general] [--no-kappa] [--no-adjoint]
@@ -145,6 +148,29 @@ def per_kernel_rates(ctx, stream, g, dr, dx):
     return out
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: CPU model, hardware threads this
+    process may use, NUMA nodes."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        threads = os.cpu_count()
+    try:
+        numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and d[4:].isdigit()])
+    except OSError:
+        numa = None
+    return {"model": model, "threads_available": threads, "numa_nodes": numa}
+
+
 def cpu_baseline(g, iters):
     """The reference's CG_ref (OpenMP build, setup/Make.GCC_OMP flags) on the
     same grid for a few iterations, rated with the same FLOP model."""
@@ -159,6 +185,7 @@ def cpu_baseline(g, iters):
     fl = hpcg_flops(level_shapes(g, g, g), out["niters"], sets=1)["total"]
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(fl / el / 1e9, 4), "unit": "GFLOP/s", "cores": cores, "kind": "reference",
+            "host": host_cpu(),
             "sample": f"CG_ref, {out['niters']} iterations + preamble at {g}^3, timed after the problem "
                       f"setup: {el:.1f} s of CG; OpenMP build with setup/Make.GCC_OMP flags; its "
                       f"Gauss-Seidel is serial (ComputeSYMGS_ref.cpp has no OpenMP)"}
@@ -344,7 +371,7 @@ def main():
     import torch
 
     import hpcg_amd
-    from hpcg_amd.models import (algorithmic_bytes_per_iteration, hpcg_flops, hpcg_rhs,
+    from hpcg_amd.models import (algorithmic_bytes_per_iteration, hpcg_bytes, hpcg_flops, hpcg_rhs,
                                  implementation_bytes_per_iteration, index_bytes_per_slot, level_shapes)
 
     g, it = args.grid, args.iters
@@ -391,6 +418,13 @@ def main():
     built_it = implementation_bytes_per_iteration(g, g, g, forms=forms)
     tbs = bytes_it * it * args.steps / el / 1e12
     built_tbs = built_it * it * args.steps / el / 1e12
+    # the reference report's "Raw Read / Write / Total B/W" (ReportResults.cpp:339-341)
+    rb = hpcg_bytes(shapes, it, sets=args.steps)
+    ref_bw = {"read_gb_s": round(rb["reads"] / el / 1e9, 1), "write_gb_s": round(rb["writes"] / el / 1e9, 1),
+              "total_gb_s": round(rb["total"] / el / 1e9, 1),
+              "rule": "ReportResults.cpp:103-135,339-341: the reference's read/write model over the CG time; "
+                      "it counts residual and post-smoother writes per nonzero and every sweep in full, "
+                      "so it runs above the HBM peak and is no roofline"}
     log(f"[bench] {args.steps} sets: {ms:.2f} ms/set, {gflops:.1f} GFLOP/s (rating with setup {rating:.1f}), "
         f"{tbs:.3f} TB/s model, {built_tbs:.3f} TB/s as built, scaled residual {res['normr'] / res['normr0']:.4e}")
 
@@ -487,6 +521,7 @@ def main():
                               "rule": "ReportResults.cpp:353, setup and optimisation /10 per set"},
         "hbm_tb_s": round(tbs, 4),
         "hbm_tb_s_as_built": round(built_tbs, 4),
+        "reference_raw_bw": ref_bw,
         "column_forms_level0": forms[0],
         "roofline": roof,
         "explicit_columns": explicit,

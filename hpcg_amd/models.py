@@ -42,6 +42,39 @@ def hpcg_flops(shapes, iters, sets=1, ref_iters=None, pre=1, post=1):
     return dict(ddot=ddot, waxpby=waxpby, spmv=spmv, mg=mg, total=total)
 
 
+def hpcg_bytes(shapes, iters, sets=1, pre=1, post=1, int_bytes=4):
+    """Bytes read and written by the reference's memory model
+    (ReportResults.cpp:103-135) for `sets` CG sets of `iters` iterations;
+    int_bytes is sizeof(local_int_t).  Its "Raw Read / Write / Total B/W"
+    (:339-341) divide these by the CG time.  The model counts the fine-grid
+    residual's and the post-smoother's writes once per stored nonzero
+    (:124,128) and every sweep in full, so it exceeds what any implementation
+    moves: it is reported for comparison with the reference's report, never
+    as a roofline."""
+    d = 8.0
+    vi = d + int_bytes
+    fsets, fit = float(sets), float(sets) * iters
+    nrow, nnz = float(shapes[0][0]), float(shapes[0][1])
+    reads = (3.0 * fit + fsets) * 2.0 * nrow * d          # dots: two vectors
+    writes = (3.0 * fit + fsets) * d                       # dots: the scalar
+    reads += (3.0 * fit + fsets) * 2.0 * nrow * d         # WAXPBY
+    writes += (3.0 * fit + fsets) * nrow * d
+    reads += (fit + fsets) * (nnz * vi + nrow * d)         # SpMV
+    writes += (fit + fsets) * nrow * d
+    for n_l, m_l, _ in shapes[:-1]:
+        n_l, m_l = float(n_l), float(m_l)
+        reads += pre * fit * (2.0 * m_l * vi + n_l * d)    # pre-smoother
+        writes += pre * fit * n_l * d
+        reads += fit * (m_l * vi + n_l * d)                # residual for the restriction
+        writes += fit * m_l * d
+        reads += post * fit * (2.0 * m_l * vi + n_l * d)   # post-smoother
+        writes += post * fit * m_l * d
+    n_c, m_c = float(shapes[-1][0]), float(shapes[-1][1])
+    reads += fit * (2.0 * m_c * vi + n_c * d)              # the coarsest level's sweep
+    writes += fit * n_c * d
+    return dict(reads=reads, writes=writes, total=reads + writes)
+
+
 def mxp_rate(shapes, iters, seconds):
     """HPCG-MxP-style rate of one mixed-precision solve: the operations the
     HPCG model (ReportResults.cpp:81-99) credits for one set of `iters`

@@ -53,6 +53,26 @@ def test_flop_model_matches_reference_closed_form():
     assert per_it / 1e9 == pytest.approx(2.618, abs=5e-4)
 
 
+def test_memory_model_matches_the_reference_report():
+    """hpcg_bytes restates the reference's read/write model
+    (ReportResults.cpp:103-135).  Pinned against a report the reference's own
+    ReportResults wrote (the reference driver with the drop-in hooks at 192^3,
+    one set of 50 iterations): Raw Read B/W=7940.07, Raw Write B/W=1834.94
+    GB/s over Benchmark Time Summary::Total=0.101896 s, and Floating Point
+    Operations Summary::Total=1.31322e+11 for the FLOP model beside it; each
+    printed to 6 significant digits."""
+    from hpcg_amd.models import hpcg_bytes, hpcg_flops, level_shapes
+    sh = level_shapes(192, 192, 192)
+    b = hpcg_bytes(sh, iters=50, sets=1)
+    t = 0.101896
+    assert b["reads"] / t / 1e9 == pytest.approx(7940.07, rel=2e-5)
+    assert b["writes"] / t / 1e9 == pytest.approx(1834.94, rel=2e-5)
+    assert hpcg_flops(sh, iters=50, sets=1)["total"] == pytest.approx(1.31322e11, rel=1e-5)
+    # a mistake the pin catches: post-smoother writes per row, not per nonzero
+    wrong = b["writes"] - 50 * sum(s[1] - s[0] for s in sh[:-1]) * 8.0
+    assert wrong / t / 1e9 != pytest.approx(1834.94, rel=1e-3)
+
+
 def test_mxp_rate_counts_the_solves_iterations():
     """The mixed solvers' HPCG-MxP-style rate credits the HPCG model's
     operations for exactly the iterations the solve took (plus the set's
