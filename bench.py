@@ -41,6 +41,7 @@ Other workloads print their own JSON line:
     --case kappa     384^3 leg alone
     --case adjoint   128^3 adjoint leg alone, plus the oracle's FP64 CPU gradient
     --case general   the general-layout leg alone
+    --case cpu       the reference CPU build to relative residual 1e-9 (no GPU)
 """
 import argparse
 import json
@@ -349,7 +350,7 @@ def main():
     ap.add_argument("--no-kappa", action="store_true", help="skip the 384^3 variable-coefficient leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint leg")
     ap.add_argument("--cpu-iters", type=int, default=20)
-    ap.add_argument("--case", default="hpcg", choices=("hpcg", "mixed", "kappa", "adjoint", "general"))
+    ap.add_argument("--case", default="hpcg", choices=("hpcg", "mixed", "kappa", "adjoint", "general", "cpu"))
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="first copy 1 GiB with a plain copy kernel (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -366,6 +367,8 @@ def main():
         return case_adjoint(args)
     if args.case == "general":
         return case_general(args)
+    if args.case == "cpu":
+        return case_cpu(args)
 
     import numpy as np
     import torch
@@ -618,6 +621,34 @@ def leg_general(stream, g=128):
     res["general_upload_s"] = round(upload_s, 2)
     res["grid"] = [g, g, g]
     return res
+
+
+def case_cpu(args):
+    """The reference CPU build alone, to relative residual 1e-9 as the
+    reference's own CG_ref measures it (CG_ref.cpp:100, recursive residual;
+    natural-order Gauss-Seidel, so more iterations than the device's
+    colouring needs): the CPU side of the time-to-solution comparison.  Takes
+    tens of seconds at 192^3, so it is not part of the default line."""
+    from hpcg_amd.models import hpcg_flops, level_shapes
+    from oracle import pyoracle as po
+    g = args.grid
+    if not po.ref_available(omp=True):
+        print(json.dumps({"metric": METRIC, "error": "reference CPU build not present (make -C oracle ref)"}))
+        return 1
+    R = po.RefProblem(g, g, g, omp=True)
+    t = time.perf_counter()
+    out = R.cg(max_iter=500, tol=1e-9)
+    el = time.perf_counter() - t
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    fl = hpcg_flops(level_shapes(g, g, g), out["niters"], sets=1)["total"]
+    res = {"metric": METRIC, "config": {"workload": f"reference CG_ref (OpenMP build, setup/Make.GCC_OMP flags) "
+                                                    f"to relative residual 1e-9 at {g}^3", "grid": [g, g, g]},
+           "value": round(el, 3), "unit": "s", "higher_is_better": False, "iterations": out["niters"],
+           "relative_residual": out["normr"] / out["normr0"], "gflops": round(fl / el / 1e9, 4),
+           "cores": cores, "host": host_cpu()}
+    log(f"[bench cpu {g}^3] CG_ref to 1e-9: {out['niters']} iterations, {el:.2f} s on {cores} threads")
+    print(json.dumps(res), flush=True)
+    return 0
 
 
 def case_general(args):
