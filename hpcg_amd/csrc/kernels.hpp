@@ -57,7 +57,10 @@ __device__ __forceinline__ int64_t tile_id() {
 // SIMD, so a wave holds the column, value and gather loads of all its slots
 // in flight instead of the default's batches of about 9 at six waves per
 // SIMD (measured with one value slot per load: 3 % faster at 512^3, the CG
-// SpMV 9 %, 1.7 % at 192^3; eight waves per SIMD was 10 % slower).  With
+// SpMV 9 %, 1.7 % at 192^3; eight waves per SIMD was 10 % slower; with
+// paired values the explicit-column 192^3 set ran 1058 GFLOP/s at two, 1047
+// at four and 1022 at eight, the general 128^3 layout 776, 779 and 670;
+// HPCGX_FP64_WAVES overrides it at build time).  With
 // compact columns (the default) a uniform slice needs no index load, all
 // loads of a row still issue before the first wait (114 VGPRs for the fine
 // step with paired values), and no limit is best: measured in alternating
@@ -71,9 +74,13 @@ __device__ __forceinline__ int64_t tile_id() {
 #ifndef HPCGX_CI_FP64_WAVES
 #define HPCGX_CI_FP64_WAVES 8
 #endif
+#ifndef HPCGX_FP64_WAVES
+#define HPCGX_FP64_WAVES 2
+#endif
 template <class VT, bool CI = false, int SLOTS = 27>
 struct SparseOccupancy {
-  static constexpr unsigned max_waves = (sizeof(VT) == 8 && SLOTS >= 8) ? (CI ? HPCGX_CI_FP64_WAVES : 2) : 8;
+  static constexpr unsigned max_waves =
+      (sizeof(VT) == 8 && SLOTS >= 8) ? (CI ? HPCGX_CI_FP64_WAVES : HPCGX_FP64_WAVES) : 8;
 };
 #define HPCGX_SPARSE_OCCUPANCY(VT, CI) __attribute__((amdgpu_waves_per_eu(1, SparseOccupancy<VT, CI>::max_waves)))
 
