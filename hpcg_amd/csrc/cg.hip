@@ -55,6 +55,34 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict
   p[i] = first ? zi + 0.0 * zi : zi + scal[S_BETA] * p[i];
 }
 
+// The fp64 solver's p update with the previous iteration's x update folded
+// in, so the old p is read once for both (CG.cpp:109-115,121):
+// x += alpha p where S_XGO is set (the previous r update ran), then
+// p = z + beta p.  first: p = z + 0.0 z (the WAXPBY(1,z,0,z) of CG.cpp:109),
+// with no x update pending.  k_cg_x applies the last iteration's x update
+// after the loop.
+__global__ __launch_bounds__(kBlock) void k_cg_px(int64_t n, const double *__restrict__ z, double *__restrict__ p,
+                                                  double *__restrict__ x, const double *__restrict__ scal,
+                                                  int first) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double zi = z[i];
+  if (first) {
+    p[i] = zi + 0.0 * zi;
+    return;
+  }
+  const double pi = p[i];
+  if (scal[S_XGO] != 0.0) x[i] = x[i] + scal[S_ALPHA] * pi;
+  p[i] = zi + scal[S_BETA] * pi;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_x(int64_t n, const double *__restrict__ p, double *__restrict__ x,
+                                                 const double *__restrict__ scal) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || scal[S_XGO] == 0.0) return;
+  x[i] = x[i] + scal[S_ALPHA] * p[i];
+}
+
 // Ap = A p and the partials of pAp = p.Ap.  One 256-row tile per block.
 // CI: column indices from the compact form.
 template <bool CI>
@@ -119,6 +147,40 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
       x[i] = xv[j] + alpha * pv[j];
       r[i] = ri;
       if (R32) r32[i] = rscale ? (float)ri * sv[j] : (float)ri;
+      acc += ri * ri;
+    }
+  }
+  const double v[1] = {acc};
+  block_partials<1>(v, partials);
+}
+
+// The fp64 solver's r update (CG.cpp:122-123): r -= alpha Ap and the
+// partials of r.r, with the chunks and reduction tree of k_cg_update (so the
+// same bits).  It records in S_XGO whether the stop test let it run; the
+// matching x update comes with the next p update (k_cg_px) or after the
+// loop (k_cg_x).
+__global__ __launch_bounds__(kBlock) void k_cg_update_r(int64_t n, double *__restrict__ r,
+                                                        const double *__restrict__ Ap, double *partials,
+                                                        double *scal) {
+  const bool stop = scal[S_STOP] != 0.0;  // a deferred stop test has ended the solve
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_XGO] = stop ? 0.0 : 1.0;
+  if (stop) return;
+  const double malpha = -scal[S_ALPHA];
+  double rv[kChunk], qv[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    const bool in = i < n;
+    rv[j] = in ? r[i] : 0.0;
+    qv[j] = in ? Ap[i] : 0.0;
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const int64_t i = chunk_index(j);
+    const double ri = rv[j] + malpha * qv[j];
+    if (i < n) {
+      r[i] = ri;
       acc += ri * ri;
     }
   }
@@ -323,7 +385,7 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
         if ((rc = finish(c, 1, G, j == 1 ? F_RZ_FIRST : F_RZ))) return rc;
       }
       tm.mark(2);
-      k_cg_p<double><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, c->scal, j == 1);
+      k_cg_px<<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, x, c->scal, j == 1);
       tm.mark(3);
       if (ci0.cbase)
         k_cg_spmv_dot<true><<<T, kBlock, 0, c->stream>>>(ci0, L0.vals, p, Ap, np, c->partials);
@@ -331,7 +393,7 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
         k_cg_spmv_dot<false><<<T, kBlock, 0, c->stream>>>(ci0, L0.vals, p, Ap, np, c->partials);
       if ((rc = finish(c, 1, T, F_PAP))) return rc;
       tm.mark(4);
-      k_cg_update<false><<<G, kBlock, 0, c->stream>>>(np, x, p, r, Ap, nullptr, c->partials, c->scal);
+      k_cg_update_r<<<G, kBlock, 0, c->stream>>>(np, r, Ap, c->partials, c->scal);
       if ((rc = finish(c, 1, G, F_NORMR2, deferred ? c->hist + (j - k) : nullptr))) return rc;
       tm.mark(5);
       HPCGX_CHECK_LAUNCH(c);
@@ -352,6 +414,11 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
       if (!(normr / normr0 > tol)) break;
     }
     k = kend + 1;
+  }
+  if (k > 1) {  // the last iteration's x update
+    k_cg_x<<<tiles(np), kBlock, 0, c->stream>>>(np, p, x, c->scal);
+    HPCGX_CHECK_LAUNCH(c);
+    HPCGX_HIP(c, hipStreamSynchronize(c->stream));
   }
   *normr_out = normr;
   *normr0_out = normr0;

@@ -154,6 +154,7 @@ enum Scalar : int {
   S_DOT,
   S_DOT2,
   S_STOP,  // deferred stop test of a tolerance-0 solve: nonzero once a norm was not > 0
+  S_XGO,   // the fp64 solver's last r update ran (S_STOP was clear): its x update is due
   S_COUNT = 16
 };
 
