@@ -237,5 +237,5 @@ def implementation_bytes_per_iteration(nx, ny, nz, levels=4, v=8, s=8, forms=Non
     n0 = shapes[0][0]
     ib0 = index_bytes_per_slot(None if forms is None else forms[0])
     total += n0 * 27 * (8 + ib0) + 2 * n0 * 8                         # CG SpMV + p.Ap
-    total += 24 * n0 + 48 * n0                                         # p update; x, r update + r.r
+    total += 40 * n0 + 24 * n0                                         # x and p updates; r update + r.r
     return total
