@@ -61,19 +61,47 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const ZT *__restrict
 // p = z + beta p.  first: p = z + 0.0 z (the WAXPBY(1,z,0,z) of CG.cpp:109),
 // with no x update pending.  k_cg_x applies the last iteration's x update
 // after the loop.
+// PAIRS: two consecutive entries per thread, read and written as 16-byte
+// pairs (n, a level's padded length, is a multiple of 64; the host takes
+// this form only when z, p and x are 16-byte aligned).
+template <bool PAIRS>
 __global__ __launch_bounds__(kBlock) void k_cg_px(int64_t n, const double *__restrict__ z, double *__restrict__ p,
                                                   double *__restrict__ x, const double *__restrict__ scal,
                                                   int first) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const double zi = z[i];
-  if (first) {
-    p[i] = zi + 0.0 * zi;
+  if constexpr (!PAIRS) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double zi = z[i];
+    if (first) {
+      p[i] = zi + 0.0 * zi;
+      return;
+    }
+    const double pi = p[i];
+    if (scal[S_XGO] != 0.0) x[i] = x[i] + scal[S_ALPHA] * pi;
+    p[i] = zi + scal[S_BETA] * pi;
     return;
   }
-  const double pi = p[i];
-  if (scal[S_XGO] != 0.0) x[i] = x[i] + scal[S_ALPHA] * pi;
-  p[i] = zi + scal[S_BETA] * pi;
+  const int64_t i = 2 * ((int64_t)blockIdx.x * kBlock + threadIdx.x);
+  if (i >= n) return;
+  const f64x2_t zi = *reinterpret_cast<const f64x2_t *>(z + i);
+  f64x2_t pn;
+  if (first) {
+    pn.x = zi.x + 0.0 * zi.x;
+    pn.y = zi.y + 0.0 * zi.y;
+  } else {
+    const f64x2_t pi = *reinterpret_cast<const f64x2_t *>(p + i);
+    if (scal[S_XGO] != 0.0) {
+      const double alpha = scal[S_ALPHA];
+      f64x2_t xi = *reinterpret_cast<const f64x2_t *>(x + i);
+      xi.x = xi.x + alpha * pi.x;
+      xi.y = xi.y + alpha * pi.y;
+      *reinterpret_cast<f64x2_t *>(x + i) = xi;
+    }
+    const double beta = scal[S_BETA];
+    pn.x = zi.x + beta * pi.x;
+    pn.y = zi.y + beta * pi.y;
+  }
+  *reinterpret_cast<f64x2_t *>(p + i) = pn;
 }
 
 __global__ __launch_bounds__(kBlock) void k_cg_x(int64_t n, const double *__restrict__ p, double *__restrict__ x,
@@ -367,6 +395,8 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
   if (precond && (rc = zero_vec(c, z, np))) return rc;
   if (deferred && ((rc = ensure_hist(c, kDeferBatch)) || (rc = ensure_evpool(c, times ? 6 * kDeferBatch : 0))))
     return rc;
+  const bool pairs_ok = np % 2 == 0 && ((reinterpret_cast<uintptr_t>(z) | reinterpret_cast<uintptr_t>(p) |
+                                          reinterpret_cast<uintptr_t>(x)) & 15) == 0;
   int k = 1;
   while (k <= max_iter && normr / normr0 > tol) {
     const int kend = deferred ? std::min(max_iter, k + kDeferBatch - 1) : k;
@@ -385,7 +415,10 @@ static int cg_fp64(Ctx *c, const double *b, double *x, int max_iter, double tol,
         if ((rc = finish(c, 1, G, j == 1 ? F_RZ_FIRST : F_RZ))) return rc;
       }
       tm.mark(2);
-      k_cg_px<<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, x, c->scal, j == 1);
+      if (pairs_ok)
+        k_cg_px<true><<<tiles(np / 2), kBlock, 0, c->stream>>>(np, z, p, x, c->scal, j == 1);
+      else
+        k_cg_px<false><<<tiles(np), kBlock, 0, c->stream>>>(np, z, p, x, c->scal, j == 1);
       tm.mark(3);
       if (ci0.cbase)
         k_cg_spmv_dot<true><<<T, kBlock, 0, c->stream>>>(ci0, L0.vals, p, Ap, np, c->partials);
